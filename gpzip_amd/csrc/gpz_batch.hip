@@ -1,0 +1,300 @@
+// gpz_batch.hip -- batched form (host buffers) and device-resident plans.
+//
+// A batch of uneven chunks runs as one replay of a captured HIP graph on one
+// lane: the kernels are gridded persistently over the segment/member tables,
+// whose sizes they read from device memory, so one instantiated graph serves
+// every batch that fits the lane's workspace.  This replaces the reference's
+// submitter/completer thread pair that packs up to 16 chunks per wgpu
+// submission (codec-gpu/src/gpu/batch.rs:101-141).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "gpz_internal.h"
+#include "gpz_runtime.h"
+#include "gpzip_hip.h"
+
+using namespace gpz;
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(GPZ_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace gpz {
+
+// Capture the FULL-mode kernel sequence for w on st into *exec.
+static int capture_graph(const Work &w, const Grids &g, hipStream_t st, hipGraphExec_t *exec) {
+    if (*exec) {
+        (void)hipGraphExecDestroy(*exec);
+        *exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue(w, g, MODE_FULL, st, nullptr);
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess) return fail(GPZ_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        return fail(GPZ_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    return GPZ_OK;
+}
+
+int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *const *ptrs, const uint64_t *lens) {
+    int rc;
+    if ((rc = ln->wsp.reserve(Caps{pl.P, pl.G, pl.M, pl.C, pl.out_cap}))) return rc;
+    if ((rc = ln->din.ensure(pl.in_bytes))) return rc;
+    if ((rc = ln->hin.ensure(pl.in_bytes))) return rc;
+    HIPCHK(hipSetDevice(ctx_device(ctx)));
+    uint8_t *h = (uint8_t *)ln->hin.p;
+    for (size_t c = 0; c < pl.C; c++)
+        if (lens[c]) memcpy(h + pl.chunk_dev_off[c], ptrs[c], lens[c]);
+    HIPCHK(hipMemcpyAsync(ln->din.p, ln->hin.p, pl.in_end, hipMemcpyHostToDevice, ln->st));
+    if ((rc = ln->wsp.upload_plan(pl, ln->st))) return rc;
+    Work w = ln->wsp.work;
+    w.in = (const uint8_t *)ln->din.p;
+    w.gzip = 1;
+    w.strict = 0;
+    Grids g = grids_for(ln->wsp.caps, ctx_cus(ctx));
+    if (!ln->graph || ln->graph_gen != ln->wsp.generation || ln->graph_in != ln->din.p) {
+        if ((rc = capture_graph(w, g, ln->st, &ln->graph))) return rc;
+        ln->graph_gen = ln->wsp.generation;
+        ln->graph_in = ln->din.p;
+    }
+    HIPCHK(hipGraphLaunch(ln->graph, ln->st));
+    return GPZ_OK;
+}
+
+int lane_batch_collect(Lane *ln, const Plan &pl, std::vector<uint8_t> &out, std::vector<uint64_t> &off,
+                       std::vector<uint64_t> &len) {
+    off.resize(pl.C);
+    len.resize(pl.C);
+    uint64_t total = 0;
+    const Work &w = ln->wsp.work;
+    HIPCHK(hipMemcpyAsync(off.data(), w.chunk_off, pl.C * 8, hipMemcpyDeviceToHost, ln->st));
+    HIPCHK(hipMemcpyAsync(len.data(), w.chunk_len, pl.C * 8, hipMemcpyDeviceToHost, ln->st));
+    HIPCHK(hipMemcpyAsync(&total, w.total, 8, hipMemcpyDeviceToHost, ln->st));
+    HIPCHK(hipStreamSynchronize(ln->st));
+    int rc = ln->hout.ensure(total + 8);
+    if (rc) return rc;
+    if (total) {
+        HIPCHK(hipMemcpyAsync(ln->hout.p, w.out, total, hipMemcpyDeviceToHost, ln->st));
+        HIPCHK(hipStreamSynchronize(ln->st));
+    }
+    out.assign((uint8_t *)ln->hout.p, (uint8_t *)ln->hout.p + total);
+    return GPZ_OK;
+}
+
+}  // namespace gpz
+
+struct gpz_batch {
+    gpz_ctx *ctx = nullptr;
+    Lane *lane = nullptr;
+    Plan pl;
+    std::vector<uint8_t> out;
+    std::vector<uint64_t> off, len;
+    int status = GPZ_OK;
+    bool done = false;
+};
+
+extern "C" {
+
+int gpz_batch_submit(gpz_ctx *ctx, const uint8_t *base, const gpz_desc *descs, size_t n, gpz_batch **out) {
+    clear_error();
+    if (!ctx || !out || (n && (!descs || !base))) return fail(GPZ_E_INVAL, "bad argument");
+    *out = nullptr;
+    std::vector<uint64_t> lens(n);
+    for (size_t i = 0; i < n; i++) lens[i] = descs[i].len;
+    auto b = new gpz_batch();
+    b->ctx = ctx;
+    int rc = build_plan(b->pl, lens.data(), nullptr, n, ctx_sub_chunk(ctx), 1);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    if (n == 0) {
+        b->done = true;
+        *out = b;
+        return GPZ_OK;
+    }
+    Lane *ln = ctx_acquire(ctx);
+    b->lane = ln;
+    std::vector<const uint8_t *> ptrs(n);
+    for (size_t c = 0; c < n; c++) ptrs[c] = base + descs[c].offset;
+    rc = lane_batch_launch(ctx, ln, b->pl, ptrs.data(), lens.data());
+    if (rc) {
+        (void)hipStreamSynchronize(ln->st);
+        ctx_release(ctx, ln);
+        b->lane = nullptr;
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return GPZ_OK;
+}
+
+int gpz_batch_wait(gpz_batch *b) {
+    if (!b) return fail(GPZ_E_INVAL, "null batch");
+    if (b->done) return b->status;
+    Lane *ln = b->lane;
+    const Plan &pl = b->pl;
+    int rc = lane_batch_collect(ln, pl, b->out, b->off, b->len);
+    ctx_release(b->ctx, ln);
+    b->lane = nullptr;
+    b->done = true;
+    b->status = rc;
+    return rc;
+}
+
+int gpz_batch_result(gpz_batch *b, size_t i, const uint8_t **data, size_t *len) {
+    if (!b || !data || !len) return fail(GPZ_E_INVAL, "bad argument");
+    if (!b->done) return fail(GPZ_E_INVAL, "batch not waited");
+    if (b->status) return b->status;
+    if (i >= b->pl.C) return fail(GPZ_E_INVAL, "chunk index %zu out of range", i);
+    *data = b->out.data() + b->off[i];
+    *len = b->len[i];
+    return GPZ_OK;
+}
+
+void gpz_batch_free(gpz_batch *b) {
+    if (!b) return;
+    if (!b->done) gpz_batch_wait(b);
+    delete b;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// device-resident plans
+
+struct gpz_plan {
+    gpz_ctx *ctx = nullptr;
+    Plan pl;
+    Lane lane;
+    uint32_t flags = 0;
+    Grids grids;
+    const void *bound_in = nullptr;
+    void *bound_out = nullptr;
+    bool timed = false;
+};
+
+extern "C" {
+
+int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flags, const gpz_tune *tune,
+                    gpz_plan **out) {
+    clear_error();
+    if (!ctx || !out || (n && !descs)) return fail(GPZ_E_INVAL, "bad argument");
+    *out = nullptr;
+    if (n == 0) return fail(GPZ_E_INVAL, "empty plan");
+    std::vector<uint64_t> lens(n), offs(n);
+    for (size_t i = 0; i < n; i++) {
+        lens[i] = descs[i].len;
+        offs[i] = descs[i].offset;
+    }
+    auto p = new gpz_plan();
+    p->ctx = ctx;
+    p->flags = flags;
+    int rc = build_plan(p->pl, lens.data(), offs.data(), n, ctx_sub_chunk(ctx), 1);
+    if (!rc) rc = p->lane.init(ctx_device(ctx));
+    if (!rc) {
+        const Plan &pl = p->pl;
+        Caps c{pl.P, pl.G, pl.M, pl.C, 256};  // output goes to the caller's buffer
+        rc = p->lane.wsp.reserve(c);
+    }
+    if (!rc) rc = p->lane.wsp.upload_plan(p->pl, p->lane.st);
+    if (!rc && hipStreamSynchronize(p->lane.st) != hipSuccess) rc = fail(GPZ_E_DEVICE, "plan upload failed");
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    p->grids = grids_for(p->lane.wsp.caps, ctx_cus(ctx));
+    if (!(flags & GPZ_PLAN_PERSISTENT)) {
+        // one workgroup per segment / member
+        p->grids.k1 = p->grids.k3 = std::max<uint32_t>(1, p->pl.G);
+        p->grids.emit = p->grids.huff = p->grids.chain = std::max<uint32_t>(1, p->pl.M);
+    }
+    if (tune) {
+        if (tune->k1_grid) p->grids.k1 = tune->k1_grid;
+        if (tune->k3_grid) p->grids.k3 = tune->k3_grid;
+        if (tune->emit_grid) p->grids.emit = tune->emit_grid;
+    }
+    *out = p;
+    return GPZ_OK;
+}
+
+size_t gpz_plan_out_capacity(const gpz_plan *p) { return p ? p->pl.out_cap : 0; }
+
+int gpz_plan_run(gpz_plan *p, const void *d_in, void *d_out) {
+    clear_error();
+    if (!p || !d_in || !d_out) return fail(GPZ_E_INVAL, "bad argument");
+    if (((uintptr_t)d_in & 15) || ((uintptr_t)d_out & 3)) return fail(GPZ_E_INVAL, "misaligned device buffer");
+    Lane &ln = p->lane;
+    HIPCHK(hipSetDevice(ctx_device(p->ctx)));
+    Work w = ln.wsp.work;
+    w.in = (const uint8_t *)d_in;
+    w.out = (uint8_t *)d_out;
+    w.gzip = 1;
+    w.strict = 0;
+    if (p->flags & GPZ_PLAN_GRAPH) {
+        if (!ln.graph || p->bound_in != d_in || p->bound_out != d_out) {
+            int rc = capture_graph(w, p->grids, ln.st, &ln.graph);
+            if (rc) return rc;
+            p->bound_in = d_in;
+            p->bound_out = d_out;
+        }
+        HIPCHK(hipEventRecord(ln.ev[0], ln.st));
+        HIPCHK(hipGraphLaunch(ln.graph, ln.st));
+        HIPCHK(hipEventRecord(ln.ev[2], ln.st));
+        p->timed = false;
+    } else {
+        enqueue(w, p->grids, MODE_FULL, ln.st, ln.ev);
+        HIPCHK(hipGetLastError());
+        p->timed = true;
+    }
+    return GPZ_OK;
+}
+
+int gpz_plan_sync(gpz_plan *p) {
+    if (!p) return fail(GPZ_E_INVAL, "null plan");
+    HIPCHK(hipStreamSynchronize(p->lane.st));
+    return GPZ_OK;
+}
+
+int gpz_plan_result(gpz_plan *p, uint64_t *chunk_off, uint64_t *chunk_len, uint64_t *total) {
+    if (!p) return fail(GPZ_E_INVAL, "null plan");
+    const Work &w = p->lane.wsp.work;
+    hipStream_t st = p->lane.st;
+    if (chunk_off) HIPCHK(hipMemcpyAsync(chunk_off, w.chunk_off, p->pl.C * 8, hipMemcpyDeviceToHost, st));
+    if (chunk_len) HIPCHK(hipMemcpyAsync(chunk_len, w.chunk_len, p->pl.C * 8, hipMemcpyDeviceToHost, st));
+    if (total) HIPCHK(hipMemcpyAsync(total, w.total, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return GPZ_OK;
+}
+
+int gpz_plan_timing(gpz_plan *p, float *ms, size_t n) {
+    if (!p || !ms) return fail(GPZ_E_INVAL, "bad argument");
+    Lane &ln = p->lane;
+    float v[3] = {0, 0, 0};
+    HIPCHK(hipEventElapsedTime(&v[0], ln.ev[0], ln.ev[2]));
+    if (p->timed) {
+        HIPCHK(hipEventElapsedTime(&v[1], ln.ev[0], ln.ev[1]));
+        HIPCHK(hipEventElapsedTime(&v[2], ln.ev[3], ln.ev[4]));
+    }
+    for (size_t i = 0; i < n && i < 3; i++) ms[i] = v[i];
+    return GPZ_OK;
+}
+
+void gpz_plan_destroy(gpz_plan *p) {
+    if (!p) return;
+    (void)hipSetDevice(ctx_device(p->ctx));
+    (void)hipStreamSynchronize(p->lane.st);
+    delete p;
+}
+
+}  // extern "C"

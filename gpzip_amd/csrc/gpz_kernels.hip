@@ -1,0 +1,1207 @@
+// gpz_kernels.hip -- gfx950 kernels of the gzip compress path.
+//
+// Pipeline per batch (one launch each, all gridded so one instance can be
+// replayed from a HIP graph for any batch that fits the workspace):
+//   k_match_find  one 1024-thread workgroup per 4 KiB segment: exact
+//                 segmented-hash candidates for the segment's positions from an
+//                 LDS copy of its 36 KiB window, candidate verification and
+//                 extension, per-position tokens, the segment's walk summary
+//                 (exit/count for every possible entry offset, by pointer
+//                 doubling) and its raw CRC-32.
+//   k_chain       one wave per member: chains the 32 segment summaries to get
+//                 each segment's walk entry and output token base.
+//   k_walk_emit   one workgroup per segment: marks the walked positions from
+//                 the known entry (pointer doubling), compacts them into the
+//                 token stream and accumulates the member histogram.
+//   k_huffman     one wave per member: exact heap-order Huffman code lengths,
+//                 canonical codes, dynamic block header, member size and CRC.
+//   k_scan        member output offsets and per-chunk offsets/lengths.
+//   k_emit        one workgroup per member: per-token bit lengths, workgroup
+//                 scan, LDS word assembly, coalesced stores of the member.
+//
+// Reference semantics restated here (paths relative to the reference repo,
+// crates/gpzip- prefix dropped):
+//   hash3/build   codec-gpu/src/gpu/lz77_hash_build.wgsl:36-54
+//   lookup        codec-gpu/src/gpu/lz77_hash_lookup.wgsl:45-119
+//   lazy walk     codec-gpu/src/gpu/lz77.rs:260-284
+//   strict walk   codec-gpu/src/gpu/walk.rs:512-525
+//   Huffman       codec-gpu/src/gpu/huffman.rs:26-152
+//   block writer  codec-gpu/src/gpu/deflate.rs:254-335, 752-959
+//   gzip framing  codec-gpu/src/gpu/deflate.rs:962-980
+#include <hip/hip_runtime.h>
+
+#include "gpz_internal.h"
+
+namespace gpz {
+
+// ---------------------------------------------------------------------------
+// constants
+
+__constant__ uint32_t c_crc_tab[256];   // reflected CRC-32 byte table
+__constant__ uint32_t c_crc_fold[256];  // x^(8*16*(255-t)) mod P, reflected
+__constant__ uint32_t c_xpow[32];       // x^(8*2^k) mod P, reflected
+
+__constant__ uint16_t c_len_base[29] = {3,  4,  5,  6,  7,  8,  9,  10, 11,  13,  15,  17,  19,  23, 27,
+                                        31, 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
+__constant__ uint16_t c_dist_base[30] = {1,    2,    3,    4,    5,    7,    9,    13,    17,    25,
+                                         33,   49,   65,   97,   129,  193,  257,  385,   513,   769,
+                                         1025, 1537, 2049, 3073, 4097, 6145, 8193, 12289, 16385, 24577};
+__constant__ uint8_t c_meta_order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+
+constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+
+// ---------------------------------------------------------------------------
+// small helpers
+
+// Length 3..258 -> index 0..28 of the RFC 1951 length rows (symbol 257+idx).
+__device__ __forceinline__ uint32_t len_idx(uint32_t len) {
+    if (len <= 10) return len - 3;
+    if (len == 258) return 28;
+    uint32_t l = len - 3;
+    uint32_t k = 31 - __clz(l);
+    return 4 * (k - 1) + ((l >> (k - 2)) & 3);
+}
+__device__ __forceinline__ uint32_t len_extra(uint32_t idx) {
+    return (idx < 8 || idx == 28) ? 0 : (idx - 4) >> 2;
+}
+// Distance 1..32768 -> symbol 0..29.
+__device__ __forceinline__ uint32_t dist_sym(uint32_t d) {
+    if (d <= 4) return d - 1;
+    uint32_t x = d - 1;
+    uint32_t k = 31 - __clz(x);
+    return 2 * k + ((x >> (k - 1)) & 1);
+}
+__device__ __forceinline__ uint32_t dist_extra(uint32_t sym) { return sym < 4 ? 0 : (sym >> 1) - 1; }
+
+// Carry-less multiply modulo the CRC-32 polynomial, reflected representation
+// (bit 31 holds the x^0 coefficient).
+__device__ __forceinline__ uint32_t gf2_mul(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+        r ^= b & (0u - ((a >> (31 - i)) & 1u));
+        b = (b >> 1) ^ (0xEDB88320u & (0u - (b & 1u)));
+    }
+    return r;
+}
+// x^(8n) mod P
+__device__ __forceinline__ uint32_t xpow8(uint64_t n) {
+    uint32_t r = 0x80000000u;
+    for (int k = 0; n; k++, n >>= 1)
+        if (n & 1) r = gf2_mul(r, c_xpow[k]);
+    return r;
+}
+
+// Workgroup exclusive scan (u32) for blocks of BLOCK threads (multiple of 64).
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *tmp, uint32_t *total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) tmp[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        uint32_t t = lane < BLOCK / 64 ? tmp[lane] : 0;
+#pragma unroll
+        for (int d = 1; d < BLOCK / 64; d <<= 1) {
+            uint32_t y = __shfl_up(t, d, 64);
+            if (lane >= d) t += y;
+        }
+        if (lane < BLOCK / 64) tmp[lane] = t;
+    }
+    __syncthreads();
+    uint32_t base = wid ? tmp[wid - 1] : 0;
+    *total = tmp[BLOCK / 64 - 1];
+    __syncthreads();
+    return base + x - v;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ uint64_t block_scan64(uint64_t v, uint64_t *tmp, uint64_t *total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) tmp[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        uint64_t t = lane < BLOCK / 64 ? tmp[lane] : 0;
+#pragma unroll
+        for (int d = 1; d < BLOCK / 64; d <<= 1) {
+            uint64_t y = __shfl_up(t, d, 64);
+            if (lane >= d) t += y;
+        }
+        if (lane < BLOCK / 64) tmp[lane] = t;
+    }
+    __syncthreads();
+    uint64_t base = wid ? tmp[wid - 1] : 0;
+    *total = tmp[BLOCK / 64 - 1];
+    __syncthreads();
+    return base + x - v;
+}
+
+// Spread consecutive work items of a persistent grid so that each XCD (blocks
+// are dealt round-robin to the 8 XCDs) walks a contiguous range: neighbouring
+// segments share 8/9 of their window, so they should share an L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+    uint32_t b = blockIdx.x, P = gridDim.x;
+    if ((P & 7u) == 0) return (b & 7u) * (P >> 3) + (b >> 3);
+    return b;
+}
+
+// Bytes x..x+3 of an LDS byte array stored as u32 words.
+__device__ __forceinline__ uint32_t ld4(const uint32_t *win, uint32_t x) {
+    uint32_t w = x >> 2;
+    return __builtin_amdgcn_alignbyte(win[w + 1], win[w], x & 3u);
+}
+
+// ---------------------------------------------------------------------------
+// walk summary (shared by k_match_find and k_walk_summary)
+//
+// For every entry offset x of the segment, the first walk position at or
+// beyond the segment end (relative to the segment start) and the number of
+// tokens emitted before it.  When the segment is not the member's last, the
+// lazy rule at its last position depends on the next segment's first token;
+// that position is made absorbing (J = x, C = 0) and k_chain resolves it.
+
+template <int BLOCK, int PER>
+__device__ void walk_summary(const uint32_t *tokL, uint16_t *Ja, uint16_t *Ca, uint16_t *Jb,
+                             uint16_t *Cb, uint32_t L, bool last_seg, uint32_t strict,
+                             uint32_t *out /* SPAN entries */) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t absorb = (!strict && !last_seg) ? L - 1 : 0xFFFFu;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        uint32_t x = tid * PER + i;
+        if (x >= L) continue;
+        uint32_t t = tokL[x], len = t >> 16, J, C = 1;
+        if (len == 0) {
+            J = x + 1;
+        } else if (strict) {
+            J = x + len;
+        } else if (x + 1 < L) {
+            uint32_t nl = tokL[x + 1] >> 16;
+            J = (nl != 0 && nl > len) ? x + 1 : x + len;
+        } else if (last_seg) {
+            J = x + len;
+        } else {
+            J = x;  // absorbing
+            C = 0;
+        }
+        Ja[x] = (uint16_t)(J < 0xFFFFu ? J : 0xFFFFu);
+        Ca[x] = (uint16_t)C;
+    }
+    __syncthreads();
+    for (int r = 0; r < 13; r++) {
+        int alive = 0;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            uint32_t x = tid * PER + i;
+            if (x >= L) continue;
+            uint32_t j = Ja[x], c = Ca[x];
+            if (j < L) {
+                c += Ca[j];
+                j = Ja[j];
+            }
+            Jb[x] = (uint16_t)j;
+            Cb[x] = (uint16_t)c;
+            alive |= (j < L && j != absorb);
+        }
+        int any = __syncthreads_or(alive);
+        uint16_t *t;
+        t = Ja; Ja = Jb; Jb = t;
+        t = Ca; Ca = Cb; Cb = t;
+        if (!any) break;
+    }
+    for (uint32_t x = tid; x < L && x < SPAN; x += BLOCK) out[x] = (uint32_t)Ja[x] | ((uint32_t)Ca[x] << 16);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// k_match_find
+
+constexpr int K1_T = 1024;
+constexpr int WIN_WORDS = 9344;  // 37,376 bytes: 32 KiB back + 4 KiB segment + 258 + slack
+constexpr int KSLOTS = 8192;
+
+struct MatchState {
+    uint32_t len, dist;
+};
+
+__device__ __forceinline__ void try_cand(const uint32_t *win, uint32_t WB, uint32_t n, uint32_t p,
+                                         uint32_t cand, uint32_t window, MatchState &st) {
+    if (cand >= p) return;
+    uint32_t dist = p - cand;
+    if (dist > window) return;
+    uint32_t xp = p - WB, xc = cand - WB;
+    uint32_t a = ld4(win, xp), b = ld4(win, xc);
+    if ((a ^ b) & 0x00FFFFFFu) return;
+    uint32_t lim = n - p;
+    if (lim > MAX_MATCH) lim = MAX_MATCH;
+    uint32_t len = 3;
+    uint32_t d = (a ^ b);
+    if (d) {
+        len = __builtin_ctz(d) >> 3;
+    } else {
+        len = 4;
+        while (len < lim) {
+            uint32_t e = ld4(win, xp + len) ^ ld4(win, xc + len);
+            if (e) {
+                len += __builtin_ctz(e) >> 3;
+                break;
+            }
+            len += 4;
+        }
+    }
+    if (len > lim) len = lim;
+    if (len > st.len) {
+        st.len = len;
+        st.dist = dist;
+    }
+}
+
+__global__ __launch_bounds__(K1_T) void k_match_find(Work w, int do_summary) {
+    __shared__ uint32_t win[WIN_WORDS];
+    __shared__ uint32_t keys[KSLOTS];
+    __shared__ uint32_t fl[2 * 4096];
+    __shared__ uint32_t crc_tab[256];
+    __shared__ uint32_t tmp[32];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t G = w.counts->nseg;
+    for (uint32_t i = tid; i < 256; i += K1_T) crc_tab[i] = c_crc_tab[i];
+
+    for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
+        const uint32_t m = w.seg_mem[g];
+        const MemberDesc md = w.mem[m];
+        const uint32_t n = md.n, s = g - md.seg_base;
+        const uint32_t S = s << SEG_LOG2;
+        const uint32_t E = min(S + SEG, n), L = E - S;
+        const uint32_t WB = S >= 32768u ? S - 32768u : 0u;
+        const uint32_t WE = min(n, E + 272u);
+        const uint32_t nb = WE - WB;
+        const uint8_t *src = w.in + md.in_off + WB;
+
+        // ---- window -> LDS (zero beyond the member end)
+        for (uint32_t v = tid; v < WIN_WORDS / 4; v += K1_T) {
+            uint32_t b0 = v * 16u;
+            uint4 q = make_uint4(0, 0, 0, 0);
+            if (b0 < nb) {
+                q = *reinterpret_cast<const uint4 *>(src + b0);
+                if (b0 + 16u > nb) {
+                    uint32_t rem = nb - b0;
+                    uint32_t *qq = reinterpret_cast<uint32_t *>(&q);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        uint32_t lo = 4u * j;
+                        if (lo >= rem) qq[j] = 0;
+                        else if (lo + 4 > rem) qq[j] &= (1u << (8 * (rem - lo))) - 1u;
+                    }
+                }
+            }
+            reinterpret_cast<uint4 *>(win)[v] = q;
+        }
+        for (uint32_t i = tid; i < KSLOTS; i += K1_T) keys[i] = EMPTY;
+        __syncthreads();
+
+        // ---- own positions: hashes, key table insert
+        const uint32_t x0 = tid * 4u;
+        const uint32_t lx = S - WB + x0;
+        uint64_t vv = (uint64_t)win[lx >> 2] | ((uint64_t)win[(lx >> 2) + 1] << 32);
+        uint32_t hq[4], slot[4], qid[4];
+        bool hv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint32_t x = x0 + i;
+            hv[i] = x < L && S + x + 2u < n;
+            uint32_t key = (uint32_t)((vv >> (8 * i)) & 0xFF) << 16 | (uint32_t)((vv >> (8 * i + 8)) & 0xFF) << 8 |
+                           (uint32_t)((vv >> (8 * i + 16)) & 0xFF);
+            hq[i] = (key * 0x9E3779B1u) >> 16;
+            slot[i] = 0;
+            if (hv[i]) {
+                uint32_t sl = hq[i] & (KSLOTS - 1);
+                for (;;) {
+                    uint32_t prev = atomicCAS(&keys[sl], EMPTY, hq[i]);
+                    if (prev == EMPTY || prev == hq[i]) break;
+                    sl = (sl + 1) & (KSLOTS - 1);
+                }
+                slot[i] = sl;
+            }
+        }
+        __syncthreads();
+        // dense ids for the distinct query hashes
+        {
+            uint32_t cnt = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) cnt += keys[tid * 8 + i] != EMPTY;
+            uint32_t D;
+            uint32_t id = block_scan<K1_T>(cnt, tmp, &D);
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                uint32_t kv = keys[tid * 8 + i];
+                if (kv != EMPTY) keys[tid * 8 + i] = (kv & 0xFFFFu) | (id++ << 16);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 4; i++) qid[i] = hv[i] ? keys[slot[i]] >> 16 : 0;
+            tmp[31] = D;
+        }
+        __syncthreads();
+        const uint32_t D = tmp[31];
+
+        MatchState st[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) st[i] = {0, 0};
+
+        // ---- back segments, newest segment first (lz77_hash_lookup.wgsl:85-112)
+        for (uint32_t k = 0; k <= 8 && k <= s; k++) {
+            const uint32_t j = s - k;
+            for (uint32_t i = tid; i < D; i += K1_T) {
+                fl[i] = EMPTY;
+                fl[4096 + i] = 0;
+            }
+            __syncthreads();
+            if (k == 0) {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (hv[i]) {
+                        atomicMin(&fl[qid[i]], S + x0 + i);
+                        atomicMax(&fl[4096 + qid[i]], S + x0 + i);
+                    }
+            } else {
+                const uint32_t q0 = (j << SEG_LOG2) + x0;  // segment j is full (j < s)
+                const uint32_t bx = q0 - WB;
+                uint64_t bv = (uint64_t)win[bx >> 2] | ((uint64_t)win[(bx >> 2) + 1] << 32);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    if (q0 + i + 2u >= n) continue;
+                    uint32_t key = (uint32_t)((bv >> (8 * i)) & 0xFF) << 16 |
+                                   (uint32_t)((bv >> (8 * i + 8)) & 0xFF) << 8 | (uint32_t)((bv >> (8 * i + 16)) & 0xFF);
+                    uint32_t h = (key * 0x9E3779B1u) >> 16;
+                    uint32_t sl = h & (KSLOTS - 1);
+                    for (;;) {
+                        uint32_t kv = keys[sl];
+                        if (kv == EMPTY) break;
+                        if ((kv & 0xFFFFu) == h) {
+                            uint32_t id = kv >> 16;
+                            atomicMin(&fl[id], q0 + i);
+                            atomicMax(&fl[4096 + id], q0 + i);
+                            break;
+                        }
+                        sl = (sl + 1) & (KSLOTS - 1);
+                    }
+                }
+            }
+            __syncthreads();
+            int active = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (!hv[i] || st[i].len >= MAX_MATCH) continue;
+                uint32_t f = fl[qid[i]];
+                if (f != EMPTY) {
+                    uint32_t l = fl[4096 + qid[i]];
+                    uint32_t p = S + x0 + i;
+                    try_cand(win, WB, n, p, l, w.window, st[i]);
+                    if (st[i].len < MAX_MATCH && st[i].len < OLDEST_SKIP) try_cand(win, WB, n, p, f, w.window, st[i]);
+                }
+                active |= st[i].len < MAX_MATCH;
+            }
+            if (!__syncthreads_or(active)) break;
+        }
+
+        // ---- tokens
+        uint32_t t4[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint32_t byte = (uint32_t)((vv >> (8 * i)) & 0xFF);
+            t4[i] = (hv[i] && st[i].len) ? ((st[i].len << 16) | st[i].dist) : byte;
+        }
+        uint32_t *tg = w.tok + md.pos_base + S + x0;
+        if (x0 + 4 <= L) {
+            *reinterpret_cast<uint4 *>(tg) = make_uint4(t4[0], t4[1], t4[2], t4[3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (x0 + i < L) tg[i] = t4[i];
+        }
+
+        // ---- raw CRC of the segment: 256 threads x 16 bytes, right-aligned to
+        // a virtual 4096-byte block (leading zero bytes leave a zero register)
+        {
+            uint32_t c = 0;
+            if (tid < 256) {
+                int pad = (int)SEG - (int)L;
+                for (int b = 0; b < 16; b++) {
+                    int r = (int)(tid * 16 + b) - pad;
+                    if (r < 0) continue;
+                    uint32_t x = S - WB + (uint32_t)r;
+                    uint32_t byte = (win[x >> 2] >> (8 * (x & 3))) & 0xFF;
+                    c = crc_tab[(c ^ byte) & 0xFF] ^ (c >> 8);
+                }
+                c = gf2_mul(c, c_crc_fold[tid]);
+            }
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) c ^= __shfl_xor(c, d, 64);
+            if ((tid & 63) == 0 && tid < 256) tmp[tid >> 6] = c;
+            __syncthreads();
+            if (tid == 0) w.seg_crc[g] = tmp[0] ^ tmp[1] ^ tmp[2] ^ tmp[3];
+        }
+
+        // ---- walk summary
+        if (do_summary) {
+            uint32_t *tokL = keys;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (x0 + i < L) tokL[x0 + i] = t4[i];
+            __syncthreads();
+            uint16_t *J = reinterpret_cast<uint16_t *>(fl);
+            walk_summary<K1_T, 4>(tokL, J, J + 4096, J + 8192, J + 12288, L, E == n, w.strict,
+                                  w.wsum + (uint64_t)g * SPAN);
+        } else {
+            __syncthreads();
+        }
+    }
+}
+
+// Walk summary from tokens already in w.tok (stage-level gpz_walk).
+__global__ __launch_bounds__(K1_T) void k_walk_summary(Work w) {
+    __shared__ uint32_t tokL[4096];
+    __shared__ uint16_t J[4 * 4096];
+    const uint32_t G = w.counts->nseg;
+    for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
+        const uint32_t m = w.seg_mem[g];
+        const MemberDesc md = w.mem[m];
+        const uint32_t n = md.n, s = g - md.seg_base;
+        const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
+        for (uint32_t x = threadIdx.x; x < L; x += K1_T) tokL[x] = w.tok[md.pos_base + S + x];
+        __syncthreads();
+        walk_summary<K1_T, 4>(tokL, J, J + 4096, J + 8192, J + 12288, L, E == n, w.strict,
+                              w.wsum + (uint64_t)g * SPAN);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_chain: one wave per member
+
+__global__ __launch_bounds__(64) void k_chain(Work w) {
+    __shared__ uint32_t sum[32 * SPAN];
+    __shared__ uint32_t btok[64];
+    const uint32_t M = w.counts->nmem;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
+        const MemberDesc md = w.mem[m];
+        const uint32_t nseg = md.nseg;
+        const bool cached = nseg <= 32;
+        if (cached) {
+            const uint32_t *src = w.wsum + (uint64_t)md.seg_base * SPAN;
+            for (uint32_t i = lane; i < nseg * SPAN; i += 64) sum[i] = src[i];
+            if (lane + 1 < nseg) {
+                uint64_t e = md.pos_base + ((uint64_t)(lane + 1) << SEG_LOG2);
+                btok[2 * lane] = w.tok[e - 1];
+                btok[2 * lane + 1] = w.tok[e];
+            }
+        }
+        for (uint32_t i = lane; i < NSYM; i += 64) w.hist[(uint64_t)m * NSYM + i] = 0;
+        __syncthreads();
+        if (lane == 0) {
+            uint32_t x = 0, base = 0;
+            for (uint32_t s = 0; s < nseg; s++) {
+                const uint32_t g = md.seg_base + s;
+                const uint32_t S = s << SEG_LOG2;
+                const uint32_t L = min(SEG, md.n - S);
+                w.seg_entry[g] = x;
+                w.seg_tokbase[g] = base;
+                if (x >= L) {
+                    x -= L;
+                    continue;
+                }
+                // entry offsets stay below SPAN for token lengths <= 258
+                const uint32_t xe = x < SPAN ? x : SPAN - 1;
+                uint32_t v = cached ? sum[s * SPAN + xe] : w.wsum[(uint64_t)g * SPAN + xe];
+                uint32_t jx = v & 0xFFFFu, c = v >> 16;
+                if (jx < L) {  // absorbed at the segment's last position
+                    uint32_t t, t1;
+                    if (cached) {
+                        t = btok[2 * s];
+                        t1 = btok[2 * s + 1];
+                    } else {
+                        uint64_t e = md.pos_base + S + L;
+                        t = w.tok[e - 1];
+                        t1 = w.tok[e];
+                    }
+                    uint32_t len = t >> 16, nl = t1 >> 16;
+                    uint32_t succ = (nl != 0 && nl > len) ? L : (L - 1 + len);
+                    base += c + 1;
+                    x = succ - L;
+                } else {
+                    base += c;
+                    x = jx - L;
+                }
+            }
+            w.ntok[m] = base;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_walk_emit: mark walked positions, compact tokens, histogram
+
+constexpr int K3_T = 1024;
+
+__global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
+    __shared__ uint32_t tokL[4096 + 4];
+    __shared__ uint16_t Ja[4096], Jb[4096];
+    __shared__ uint8_t mark[4096];
+    __shared__ uint32_t hist[NSYM];
+    __shared__ uint32_t tmp[32];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t G = w.counts->nseg;
+    for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
+        const uint32_t m = w.seg_mem[g];
+        const MemberDesc md = w.mem[m];
+        const uint32_t n = md.n, s = g - md.seg_base;
+        const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
+        const uint32_t e = w.seg_entry[g];
+        if (e >= L) continue;  // the walk jumps over this segment (uniform)
+        const uint32_t tokbase = w.seg_tokbase[g];
+        const uint32_t *tg = w.tok + md.pos_base + S;
+        const uint32_t x0 = tid * 4;
+        if (x0 + 4 <= L) {
+            uint4 q = *reinterpret_cast<const uint4 *>(tg + x0);
+            tokL[x0] = q.x;
+            tokL[x0 + 1] = q.y;
+            tokL[x0 + 2] = q.z;
+            tokL[x0 + 3] = q.w;
+        } else {
+            for (uint32_t x = x0; x < L; x++) tokL[x] = tg[x];
+        }
+        if (tid == 0) tokL[L] = E < n ? tg[L] : 0;
+        for (uint32_t i = tid; i < NSYM; i += K3_T) hist[i] = 0;
+        __syncthreads();
+        uint32_t dem = 0;  // bit i: position x0+i is a match demoted to a literal
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint32_t x = x0 + i;
+            if (x >= L) continue;
+            uint32_t t = tokL[x], len = t >> 16, J;
+            if (len == 0) {
+                J = x + 1;
+            } else if (w.strict) {
+                J = x + len;
+            } else if (x + 1 < n - S) {
+                uint32_t nl = tokL[x + 1] >> 16;
+                if (nl != 0 && nl > len) {
+                    J = x + 1;
+                    dem |= 1u << i;
+                } else {
+                    J = x + len;
+                }
+            } else {
+                J = x + len;
+            }
+            Ja[x] = (uint16_t)(J < 0xFFFFu ? J : 0xFFFFu);
+            mark[x] = x == e;
+        }
+        __syncthreads();
+        uint16_t *A = Ja, *B = Jb;
+        for (int r = 0; r < 13; r++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint32_t x = x0 + i;
+                if (x < L && mark[x]) {
+                    uint32_t j = A[x];
+                    if (j < L) mark[j] = 1;
+                }
+            }
+            __syncthreads();
+            int alive = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint32_t x = x0 + i;
+                if (x >= L) continue;
+                uint32_t j = A[x];
+                if (j < L) j = A[j];
+                B[x] = (uint16_t)j;
+                alive |= j < L;
+            }
+            int any = __syncthreads_or(alive);
+            uint16_t *t = A;
+            A = B;
+            B = t;
+            if (!any) break;
+        }
+        // ranks and output
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) cnt += (x0 + i < L) && mark[x0 + i];
+        uint32_t total;
+        uint32_t rank = block_scan<K3_T>(cnt, tmp, &total);
+        uint32_t *outp = w.walked + md.pos_base + tokbase;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint32_t x = x0 + i;
+            if (x >= L || !mark[x]) continue;
+            uint32_t t = tokL[x];
+            if (dem >> i & 1) t = w.in[md.in_off + S + x];
+            outp[rank++] = t;
+            uint32_t len = t >> 16, v = t & 0xFFFFu;
+            if (len == 0) {
+                if (v < 256) atomicAdd(&hist[v], 1u);
+            } else if (len >= 3 && len <= 258 && v >= 1 && v <= 32768) {
+                atomicAdd(&hist[257 + len_idx(len)], 1u);
+                atomicAdd(&hist[DSYM0 + dist_sym(v)], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < NSYM; i += K3_T)
+            if (hist[i]) atomicAdd(&w.hist[(uint64_t)m * NSYM + i], hist[i]);
+        __syncthreads();
+    }
+}
+
+// Histogram of caller-supplied walked tokens (stage-level gpz_encode_block).
+__global__ __launch_bounds__(1024) void k_hist_tokens(Work w) {
+    __shared__ uint32_t hist[NSYM];
+    const uint32_t M = w.counts->nmem;
+    for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
+        const MemberDesc md = w.mem[m];
+        for (uint32_t i = threadIdx.x; i < NSYM; i += 1024) hist[i] = 0;
+        __syncthreads();
+        const uint32_t nt = w.ntok[m];
+        for (uint32_t i = threadIdx.x; i < nt; i += 1024) {
+            uint32_t t = w.walked[md.pos_base + i], len = t >> 16, v = t & 0xFFFFu;
+            if (len == 0) {
+                atomicAdd(&hist[v & 0xFF], 1u);
+            } else {
+                atomicAdd(&hist[257 + len_idx(len)], 1u);
+                atomicAdd(&hist[DSYM0 + dist_sym(v)], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < NSYM; i += 1024) w.hist[(uint64_t)m * NSYM + i] = hist[i];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_huffman: one wave per member
+
+struct HuffScratch {
+    uint32_t work[288];
+    uint32_t ifreq[288];
+    uint16_t sorted[288];
+    uint16_t lparent[288];
+    uint16_t iparent[288];
+    uint16_t idepth[288];
+    uint32_t misc[4];
+};
+
+// try_build_code_lengths (huffman.rs:26-60).  The reference pops the minimum
+// (freq, id) from a binary heap whose internal nodes get ids n, n+1, ...;
+// merged frequencies never decrease and internal ids exceed every leaf id,
+// so merging a (freq, id)-sorted leaf list with a FIFO of internal nodes
+// (leaf first on equal frequency) pops the same sequence.
+__device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens, HuffScratch &hs) {
+    const int lane = threadIdx.x;
+    if (lane == 0) hs.misc[0] = 0;
+    __syncthreads();
+    uint32_t ucnt = 0;
+    for (int i = lane; i < n; i += 64) {
+        hs.work[i] = f[i];
+        ucnt += f[i] > 0;
+    }
+    for (int d = 32; d >= 1; d >>= 1) ucnt += __shfl_xor(ucnt, d, 64);
+    const uint32_t used = ucnt;
+    __syncthreads();
+    if (used <= 1) {
+        for (int i = lane; i < n; i += 64) lens[i] = f[i] > 0 ? 1 : 0;
+        __syncthreads();
+        return;
+    }
+    for (;;) {
+        for (int i = lane; i < n; i += 64) {
+            uint32_t wi = hs.work[i];
+            if (!wi) continue;
+            uint32_t r = 0;
+            for (int j = 0; j < n; j++) {
+                uint32_t wj = hs.work[j];
+                r += (wj != 0) & ((wj < wi) | ((wj == wi) & (j < i)));
+            }
+            hs.sorted[r] = (uint16_t)i;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            uint32_t nl = 0, qh = 0;
+            for (uint32_t k = 0; k + 1 < used; k++) {
+                uint32_t fsum = 0;
+                for (int pick = 0; pick < 2; pick++) {
+                    bool leaf = nl < used && (qh >= k || hs.work[hs.sorted[nl]] <= hs.ifreq[qh]);
+                    if (leaf) {
+                        uint32_t id = hs.sorted[nl++];
+                        fsum += hs.work[id];
+                        hs.lparent[id] = (uint16_t)k;
+                    } else {
+                        uint32_t id = qh++;
+                        fsum += hs.ifreq[id];
+                        hs.iparent[id] = (uint16_t)k;
+                    }
+                }
+                hs.ifreq[k] = fsum;
+            }
+            hs.idepth[used - 2] = 0;
+            for (int k = (int)used - 3; k >= 0; k--) hs.idepth[k] = hs.idepth[hs.iparent[k]] + 1;
+        }
+        __syncthreads();
+        uint32_t mx = 0;
+        for (int i = lane; i < n; i += 64) {
+            uint32_t d = hs.work[i] ? (uint32_t)hs.idepth[hs.lparent[i]] + 1 : 0;
+            mx = max(mx, d);
+        }
+        for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
+        if (mx <= (uint32_t)limit) {
+            for (int i = lane; i < n; i += 64)
+                lens[i] = hs.work[i] ? (uint8_t)(hs.idepth[hs.lparent[i]] + 1) : 0;
+            __syncthreads();
+            return;
+        }
+        __syncthreads();
+        for (int i = lane; i < n; i += 64)
+            if (hs.work[i] > 1) hs.work[i] >>= 1;
+        __syncthreads();
+    }
+}
+
+// canonical codes (huffman.rs:122-152), stored bit-reversed (deflate.rs:876-884)
+__device__ void canonical_rev(const uint8_t *lens, int n, uint32_t *codes) {
+    uint32_t bl_count[16], next_code[16];
+    for (int b = 0; b < 16; b++) bl_count[b] = 0;
+    for (int i = 0; i < n; i++) bl_count[lens[i]]++;
+    bl_count[0] = 0;
+    uint32_t code = 0;
+    next_code[0] = 0;
+    for (int b = 1; b < 16; b++) {
+        code = (code + bl_count[b - 1]) << 1;
+        next_code[b] = code;
+    }
+    for (int i = 0; i < n; i++) {
+        uint32_t l = lens[i];
+        if (!l) {
+            codes[i] = 0;
+            continue;
+        }
+        uint32_t c = next_code[l]++;
+        codes[i] = (__builtin_bitreverse32(c) >> (32 - l)) | (l << 16);
+    }
+}
+
+struct BitBuf {
+    uint32_t *words;
+    uint32_t nbits;
+    __device__ void put(uint32_t v, uint32_t nb) {
+        if (!nb) return;
+        uint32_t wi = nbits >> 5, sh = nbits & 31;
+        words[wi] |= v << sh;
+        if (sh + nb > 32) words[wi + 1] |= v >> (32 - sh);
+        nbits += nb;
+    }
+};
+
+__global__ __launch_bounds__(64) void k_huffman(Work w) {
+    __shared__ uint32_t freq[NSYM];
+    __shared__ uint8_t lens[NSYM];
+    __shared__ uint32_t codes[NSYM];
+    __shared__ uint8_t comb[320];
+    __shared__ uint32_t rle[320];
+    __shared__ uint32_t mfreq[20];
+    __shared__ uint8_t mlens[20];
+    __shared__ uint32_t mcodes[20];
+    __shared__ uint32_t pbuf[PREFIX_WORDS + 2];
+    __shared__ uint32_t info[8];
+    __shared__ HuffScratch hs;
+    const uint32_t M = w.counts->nmem;
+    const int lane = threadIdx.x;
+    for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
+        const MemberDesc md = w.mem[m];
+        for (int i = lane; i < (int)NSYM; i += 64) freq[i] = w.hist[(uint64_t)m * NSYM + i];
+        for (int i = lane; i < (int)PREFIX_WORDS + 2; i += 64) pbuf[i] = 0;
+        for (int i = lane; i < 20; i += 64) mfreq[i] = 0;
+        __syncthreads();
+        if (lane == 0) freq[256] += 1;  // end of block
+        __syncthreads();
+        build_lengths(freq, 286, 15, lens, hs);
+        build_lengths(freq + DSYM0, 30, 15, lens + DSYM0, hs);
+        if (lane == 0) {
+            // single/zero distance-code fix-up and litlen fix-up (deflate.rs:855-867)
+            uint32_t used_d = 0, only = 0;
+            for (int i = 0; i < 30; i++)
+                if (lens[DSYM0 + i]) {
+                    if (!used_d) only = i;
+                    used_d++;
+                }
+            if (used_d == 0) {
+                lens[DSYM0] = 1;
+                lens[DSYM0 + 1] = 1;
+            } else if (used_d == 1) {
+                lens[DSYM0 + only] = 1;
+                lens[DSYM0 + 1 - min(only, 1u)] = 1;
+            }
+            uint32_t used_ll = 0;
+            for (int i = 0; i < 286; i++) used_ll += lens[i] > 0;
+            if (used_ll < 2) {
+                lens[0] = 1;
+                lens[256] = 1;
+            }
+            canonical_rev(lens, 286, codes);
+            canonical_rev(lens + DSYM0, 30, codes + DSYM0);
+            // trim_to_min (deflate.rs:329-335) and RLE (deflate.rs:254-326)
+            uint32_t nlit = 286, ndist = 30;
+            while (nlit > 257 && lens[nlit - 1] == 0) nlit--;
+            while (ndist > 1 && lens[DSYM0 + ndist - 1] == 0) ndist--;
+            for (uint32_t i = 0; i < nlit; i++) comb[i] = lens[i];
+            for (uint32_t i = 0; i < ndist; i++) comb[nlit + i] = lens[DSYM0 + i];
+            const uint32_t nc = nlit + ndist;
+            uint32_t k = 0, i = 0;
+            while (i < nc) {
+                uint32_t cur = comb[i], run = 1;
+                while (i + run < nc && comb[i + run] == cur) run++;
+                if (cur == 0) {
+                    while (run >= 11) {
+                        uint32_t q = min(run, 138u);
+                        rle[k++] = 18u | (7u << 8) | ((q - 11) << 16);
+                        run -= q;
+                        i += q;
+                    }
+                    while (run >= 3) {
+                        uint32_t q = min(run, 10u);
+                        rle[k++] = 17u | (3u << 8) | ((q - 3) << 16);
+                        run -= q;
+                        i += q;
+                    }
+                    while (run > 0) {
+                        rle[k++] = 0;
+                        run--;
+                        i++;
+                    }
+                } else {
+                    rle[k++] = cur;
+                    i++;
+                    run--;
+                    while (run >= 3) {
+                        uint32_t q = min(run, 6u);
+                        rle[k++] = 16u | (2u << 8) | ((q - 3) << 16);
+                        run -= q;
+                        i += q;
+                    }
+                    while (run > 0) {
+                        rle[k++] = cur;
+                        run--;
+                        i++;
+                    }
+                }
+            }
+            for (uint32_t r = 0; r < k; r++) mfreq[rle[r] & 0xFF]++;
+            info[0] = nlit;
+            info[1] = ndist;
+            info[2] = k;
+        }
+        __syncthreads();
+        build_lengths(mfreq, 19, 7, mlens, hs);
+        if (lane == 0) {
+            canonical_rev(mlens, 19, mcodes);
+            uint32_t hclen = 19;
+            while (hclen > 4 && mlens[c_meta_order[hclen - 1]] == 0) hclen--;
+            BitBuf bb{pbuf, 0};
+            if (w.gzip) {
+                // gzip header 1f 8b 08 00 00000000 00 ff (deflate.rs:965-972)
+                bb.put(0x00088b1fu, 32);
+                bb.put(0, 32);
+                bb.put(0xff00u, 16);
+            }
+            bb.put(5u, 3);  // BFINAL=1, BTYPE=10
+            bb.put(info[0] - 257, 5);
+            bb.put(info[1] - 1, 5);
+            bb.put(hclen - 4, 4);
+            for (uint32_t i = 0; i < hclen; i++) bb.put(mlens[c_meta_order[i]], 3);
+            for (uint32_t r = 0; r < info[2]; r++) {
+                uint32_t c = rle[r] & 0xFF, eb = (rle[r] >> 8) & 0xFF, ev = rle[r] >> 16;
+                bb.put(mcodes[c] & 0xFFFF, mlens[c]);
+                bb.put(ev, eb);
+            }
+            info[3] = bb.nbits;
+        }
+        __syncthreads();
+        // publish codes / prefix, count body bits
+        uint64_t body = 0;
+        for (int i = lane; i < (int)NSYM; i += 64) {
+            uint32_t c = 0;
+            if (i < 286 || (i >= (int)DSYM0 && i < (int)DSYM0 + 30)) c = codes[i];
+            w.codes[(uint64_t)m * NSYM + i] = c;
+            uint32_t f = freq[i];
+            if (!f) continue;
+            uint32_t extra = 0;
+            if (i >= 265 && i < 285) extra = len_extra(i - 257);
+            if (i >= (int)DSYM0 && i < (int)DSYM0 + 30) extra = dist_extra(i - DSYM0);
+            body += (uint64_t)f * ((c >> 16) + extra);
+        }
+        for (int d = 32; d >= 1; d >>= 1) body += __shfl_xor(body, d, 64);
+        const uint32_t pbits = info[3];
+        for (uint32_t i = lane; i < (pbits + 31) / 32; i += 64) w.prefix[(uint64_t)m * PREFIX_WORDS + i] = pbuf[i];
+        // member CRC-32 from the segments' raw CRCs
+        uint32_t crc = 0;
+        if (w.gzip) {
+            for (uint32_t s = lane; s < md.nseg; s += 64) {
+                uint64_t end = min((uint64_t)(s + 1) << SEG_LOG2, (uint64_t)md.n);
+                crc ^= gf2_mul(w.seg_crc[md.seg_base + s], xpow8(md.n - end));
+            }
+            for (int d = 32; d >= 1; d >>= 1) crc ^= __shfl_xor(crc, d, 64);
+            crc ^= gf2_mul(0xFFFFFFFFu, xpow8(md.n)) ^ 0xFFFFFFFFu;
+        }
+        if (lane == 0) {
+            w.prefix_bits[m] = pbits;
+            uint64_t bits = pbits + body;
+            w.mem_bytes[m] = (bits + 7) / 8 + (w.gzip ? 8 : 0);
+            w.mem_crc[m] = crc;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_scan: member and chunk output offsets (single workgroup)
+
+__global__ __launch_bounds__(1024) void k_scan(Work w) {
+    __shared__ uint64_t tmp[16];
+    const uint32_t M = w.counts->nmem, C = w.counts->nchunk;
+    uint64_t carry = 0;
+    for (uint32_t b = 0; b < M; b += 1024) {
+        uint32_t i = b + threadIdx.x;
+        uint64_t v = i < M ? w.mem_bytes[i] : 0, tot;
+        uint64_t ex = block_scan64<1024>(v, tmp, &tot);
+        if (i < M) w.mem_off[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *w.total = carry;
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < C; c += 1024) {
+        uint32_t m0 = w.chunk_mem0[c], m1 = w.chunk_mem0[c + 1];
+        uint64_t o0 = w.mem_off[m0];
+        uint64_t o1 = m1 < M ? w.mem_off[m1] : carry;
+        w.chunk_off[c] = o0;
+        w.chunk_len[c] = o1 - o0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_emit: one workgroup per member
+
+constexpr int K6_T = 1024;
+constexpr int BUF_WORDS = 6160;  // 4096 tokens x 48 bits + carry
+
+struct Emitter {
+    uint32_t *buf;
+    uint8_t *out;
+    uint64_t A, B;      // member byte range
+    uint64_t cursor;    // physical bit position
+    uint32_t carry;     // bits below cursor%32 of word cursor/32
+
+    __device__ __forceinline__ void or_bits(uint32_t rel, uint32_t v, uint32_t nb) const {
+        if (!nb) return;
+        uint32_t pos = (uint32_t)(cursor & 31) + rel;
+        uint32_t wi = pos >> 5, sh = pos & 31;
+        atomicOr(&buf[wi], v << sh);
+        if (sh + nb > 32) atomicOr(&buf[wi + 1], v >> (32 - sh));
+    }
+    __device__ __forceinline__ void store_word(uint64_t wi, uint32_t v) const {
+        uint64_t b0 = wi * 4;
+        if (b0 >= A && b0 + 4 <= B) {
+            reinterpret_cast<uint32_t *>(out)[wi] = v;
+        } else {
+            for (int k = 0; k < 4; k++) {
+                uint64_t a = b0 + k;
+                if (a >= A && a < B) out[a] = (uint8_t)(v >> (8 * k));
+            }
+        }
+    }
+    // before contributions: clear the words the tile touches, seed the carry
+    __device__ __forceinline__ void begin(uint32_t tile_bits) {
+        uint32_t nw = ((uint32_t)(cursor & 31) + tile_bits + 31) >> 5;
+        for (uint32_t i = threadIdx.x; i < nw + 1; i += K6_T) buf[i] = i == 0 ? carry : 0;
+        __syncthreads();
+    }
+    // after contributions: store complete words, keep the partial one
+    __device__ __forceinline__ void end(uint32_t tile_bits) {
+        __syncthreads();
+        uint32_t off0 = (uint32_t)(cursor & 31);
+        uint32_t full = (off0 + tile_bits) >> 5;
+        uint64_t w0 = cursor >> 5;
+        for (uint32_t i = threadIdx.x; i < full; i += K6_T) store_word(w0 + i, buf[i]);
+        carry = ((off0 + tile_bits) & 31) ? buf[full] : 0;
+        cursor += tile_bits;
+        __syncthreads();
+    }
+};
+
+__global__ __launch_bounds__(K6_T) void k_emit(Work w) {
+    __shared__ uint32_t ctab[NSYM];
+    __shared__ uint32_t buf[BUF_WORDS];
+    __shared__ uint32_t tmp[32];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t M = w.counts->nmem;
+    for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
+        const MemberDesc md = w.mem[m];
+        for (uint32_t i = tid; i < NSYM; i += K6_T) ctab[i] = w.codes[(uint64_t)m * NSYM + i];
+        Emitter em;
+        em.buf = buf;
+        em.out = w.out;
+        em.A = w.mem_off[m];
+        em.B = em.A + w.mem_bytes[m];
+        em.cursor = em.A * 8;
+        em.carry = 0;
+        __syncthreads();
+
+        // prefix: gzip header + dynamic block header
+        {
+            const uint32_t pbits = w.prefix_bits[m];
+            em.begin(pbits);
+            const uint32_t *pw = w.prefix + (uint64_t)m * PREFIX_WORDS;
+            for (uint32_t i = tid; i * 32 < pbits; i += K6_T) em.or_bits(i * 32, pw[i], min(32u, pbits - i * 32));
+            em.end(pbits);
+        }
+        // tokens, 4096 per tile
+        const uint32_t nt = w.ntok[m];
+        const uint32_t *tk = w.walked + md.pos_base;
+        for (uint32_t t0 = 0; t0 < nt; t0 += 4 * K6_T) {
+            uint32_t v[4][4], nb[4][4];
+            uint32_t mybits = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                uint32_t ti = t0 + tid * 4 + k;
+#pragma unroll
+                for (int f = 0; f < 4; f++) {
+                    v[k][f] = 0;
+                    nb[k][f] = 0;
+                }
+                if (ti >= nt) continue;
+                uint32_t t = tk[ti], len = t >> 16, x = t & 0xFFFFu;
+                if (len == 0) {
+                    uint32_t c = ctab[x & 0xFF];
+                    v[k][0] = c & 0xFFFF;
+                    nb[k][0] = c >> 16;
+                } else {
+                    uint32_t li = len_idx(len);
+                    uint32_t c = ctab[257 + li];
+                    v[k][0] = c & 0xFFFF;
+                    nb[k][0] = c >> 16;
+                    v[k][1] = len - c_len_base[li];
+                    nb[k][1] = len_extra(li);
+                    uint32_t ds = dist_sym(x);
+                    uint32_t dc = ctab[DSYM0 + ds];
+                    v[k][2] = dc & 0xFFFF;
+                    nb[k][2] = dc >> 16;
+                    v[k][3] = x - c_dist_base[ds];
+                    nb[k][3] = dist_extra(ds);
+                }
+                mybits += nb[k][0] + nb[k][1] + nb[k][2] + nb[k][3];
+            }
+            uint32_t tile_bits;
+            uint32_t rel = block_scan<K6_T>(mybits, tmp, &tile_bits);
+            em.begin(tile_bits);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int f = 0; f < 4; f++) {
+                    em.or_bits(rel, v[k][f], nb[k][f]);
+                    rel += nb[k][f];
+                }
+            em.end(tile_bits);
+        }
+        // end of block, zero pad to a byte, gzip trailer
+        {
+            uint32_t eob = ctab[256];
+            uint32_t eb = eob >> 16;
+            uint32_t padded = (uint32_t)(((em.cursor + eb + 7) & ~7ull) - em.cursor);
+            uint32_t tile_bits = padded + (w.gzip ? 64 : 0);
+            em.begin(tile_bits);
+            if (tid == 0) {
+                em.or_bits(0, eob & 0xFFFF, eb);
+                if (w.gzip) {
+                    em.or_bits(padded, w.mem_crc[m], 32);
+                    em.or_bits(padded + 32, md.n, 32);
+                }
+            }
+            em.end(tile_bits);
+            if (tid == 0 && (em.cursor & 31)) em.store_word(em.cursor >> 5, em.carry);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace gpz
+
+// ---------------------------------------------------------------------------
+// launch helpers used by the host runtime
+
+namespace gpz {
+
+hipError_t upload_constants() {
+    uint32_t tab[256], fold[256], xp[32];
+    for (uint32_t i = 0; i < 256; i++) {
+        uint32_t c = i;
+        for (int k = 0; k < 8; k++) c = (c & 1u) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
+        tab[i] = c;
+    }
+    auto mul = [](uint32_t a, uint32_t b) {
+        uint32_t r = 0;
+        for (int i = 0; i < 32; i++) {
+            if (a & (0x80000000u >> i)) r ^= b;
+            b = (b & 1u) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+        }
+        return r;
+    };
+    xp[0] = 0x00800000u;  // x^8
+    for (int k = 1; k < 32; k++) xp[k] = mul(xp[k - 1], xp[k - 1]);
+    auto pw = [&](uint64_t n) {
+        uint32_t r = 0x80000000u;
+        for (int k = 0; n; k++, n >>= 1)
+            if (n & 1) r = mul(r, xp[k]);
+        return r;
+    };
+    for (uint32_t t = 0; t < 256; t++) fold[t] = pw(16ull * (255 - t));
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_tab), tab, sizeof(tab));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_fold), fold, sizeof(fold));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_xpow), xp, sizeof(xp));
+    return e;
+}
+
+void launch_match_find(const Work &w, uint32_t grid, int summary, hipStream_t st) {
+    hipLaunchKernelGGL(k_match_find, dim3(grid), dim3(K1_T), 0, st, w, summary);
+}
+void launch_walk_summary(const Work &w, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_walk_summary, dim3(grid), dim3(K1_T), 0, st, w);
+}
+void launch_chain(const Work &w, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_chain, dim3(grid), dim3(64), 0, st, w);
+}
+void launch_walk_emit(const Work &w, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_walk_emit, dim3(grid), dim3(K3_T), 0, st, w);
+}
+void launch_hist_tokens(const Work &w, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_hist_tokens, dim3(grid), dim3(1024), 0, st, w);
+}
+void launch_huffman(const Work &w, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_huffman, dim3(grid), dim3(64), 0, st, w);
+}
+void launch_scan(const Work &w, hipStream_t st) { hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, w); }
+void launch_emit(const Work &w, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_emit, dim3(grid), dim3(K6_T), 0, st, w);
+}
+
+}  // namespace gpz

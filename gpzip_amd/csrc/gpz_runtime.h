@@ -1,0 +1,114 @@
+// gpz_runtime.h -- host-side structures shared by the runtime translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "gpz_internal.h"
+
+struct gpz_ctx;
+
+namespace gpz {
+
+enum { MODE_FULL = 0, MODE_MATCH = 1, MODE_WALK = 2, MODE_ENCODE = 3 };
+
+int fail(int code, const char *fmt, ...);
+void clear_error();
+const char *last_error();
+
+struct DevBuf;
+struct HostBuf;
+
+// Chunk list expanded to members and segments.
+struct Plan {
+    std::vector<MemberDesc> mem;
+    std::vector<uint32_t> seg_mem;
+    std::vector<uint32_t> chunk_mem0;     // C + 1 entries
+    std::vector<uint64_t> chunk_dev_off;  // chunk offsets in the device input buffer
+    uint64_t P = 0, in_end = 0, in_bytes = 0, out_cap = 0;
+    uint32_t G = 0, M = 0, C = 0;
+    int gzip = 1;
+};
+
+int build_plan(Plan &pl, const uint64_t *lens, const uint64_t *dev_off, size_t C, uint32_t sub_chunk,
+               int gzip);
+
+struct Caps {
+    uint64_t P = 0, G = 0, M = 0, C = 0, out = 0;
+};
+
+struct Layout {
+    uint64_t mem, seg_mem, chunk_mem0, counts, desc_end;
+    uint64_t tok, wsum, seg_crc, seg_entry, seg_tokbase, walked, hist, ntok, codes, prefix, prefix_bits;
+    uint64_t mem_bytes, mem_crc, mem_off, chunk_off, chunk_len, total, out, end;
+};
+
+struct Grids {
+    uint32_t k1 = 1, k3 = 1, chain = 1, huff = 1, emit = 1;
+};
+
+Grids grids_for(const Caps &c, int n_cu);
+void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t *ev);
+
+}  // namespace gpz
+
+// DevBuf / HostBuf are defined in gpz_runtime.hip; the structs below hold them
+// by value, so the definitions are repeated here for the other units.
+namespace gpz {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    ~DevBuf();
+};
+
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    ~HostBuf();
+};
+
+struct Workspace {
+    DevBuf ws;
+    HostBuf hdesc;
+    Caps caps;
+    Layout lay{};
+    Work work{};
+    uint64_t generation = 0;
+    int reserve(const Caps &want);
+    int upload_plan(const Plan &pl, hipStream_t st);
+};
+
+struct Lane {
+    hipStream_t st = nullptr;
+    hipEvent_t ev[6] = {};
+    Workspace wsp;
+    DevBuf din;
+    HostBuf hin, hout;
+    hipGraphExec_t graph = nullptr;
+    uint64_t graph_gen = 0;       // workspace generation the graph was captured for
+    const void *graph_in = nullptr;
+    uint32_t strict = 0;
+    int init(int dev);
+    ~Lane();
+};
+
+int ctx_device(gpz_ctx *c);
+int ctx_cus(gpz_ctx *c);
+uint32_t ctx_sub_chunk(gpz_ctx *c);
+Lane *ctx_acquire(gpz_ctx *c);
+void ctx_release(gpz_ctx *c, Lane *l);
+int lane_run(gpz_ctx *ctx, Lane *ln, const Plan &pl, int mode, const uint8_t *host_in, const uint64_t *host_lens);
+int lane_fetch_chunk(Lane *ln, uint32_t c, uint8_t *out, size_t out_cap, size_t *out_len);
+
+// Batched step on a lane: pack the chunks into pinned staging, upload, replay
+// the lane's graph.  Returns once enqueued.
+int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *const *ptrs, const uint64_t *lens);
+// Wait for the step and copy every chunk's members to host.
+int lane_batch_collect(Lane *ln, const Plan &pl, std::vector<uint8_t> &out, std::vector<uint64_t> &off,
+                       std::vector<uint64_t> &len);
+
+}  // namespace gpz

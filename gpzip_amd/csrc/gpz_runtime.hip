@@ -1,0 +1,675 @@
+// gpz_runtime.hip -- host runtime behind include/gpzip_hip.h.
+//
+// One process drives one GPU.  A context owns a pool of lanes; a lane is a
+// HIP stream plus a workspace (device arrays laid out by capacity, so the
+// same kernel arguments -- and hence the same captured HIP graph -- serve
+// every batch that fits) and pinned staging buffers.  The plain per-chunk
+// entry point takes a free lane per call, so up to `lanes` callers overlap on
+// the device, as the reference's 16 in-flight ChunkFn calls share one wgpu
+// queue (codec-gpu/src/gpu.rs:161-165, codec-gpu/src/gpu/batch.rs:36-41).
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gpz_internal.h"
+#include "gpz_runtime.h"
+#include "gpzip_hip.h"
+
+namespace gpz {
+
+hipError_t upload_constants();
+void launch_match_find(const Work &w, uint32_t grid, int summary, hipStream_t st);
+void launch_walk_summary(const Work &w, uint32_t grid, hipStream_t st);
+void launch_chain(const Work &w, uint32_t grid, hipStream_t st);
+void launch_walk_emit(const Work &w, uint32_t grid, hipStream_t st);
+void launch_hist_tokens(const Work &w, uint32_t grid, hipStream_t st);
+void launch_huffman(const Work &w, uint32_t grid, hipStream_t st);
+void launch_scan(const Work &w, hipStream_t st);
+void launch_emit(const Work &w, uint32_t grid, hipStream_t st);
+
+static thread_local char g_err[512];
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+void clear_error() { g_err[0] = 0; }
+const char *last_error() { return g_err; }
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(GPZ_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+static inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// ---------------------------------------------------------------------------
+// buffers
+
+int DevBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return GPZ_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(round_up(bytes, 1 << 20), 1 << 20);
+    if (hipMalloc(&p, want) != hipSuccess) {
+        p = nullptr;
+        return fail(GPZ_E_OOM, "hipMalloc(%zu) failed", want);
+    }
+    cap = want;
+    return GPZ_OK;
+}
+DevBuf::~DevBuf() {
+    if (p) hipFree(p);
+}
+
+int HostBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return GPZ_OK;
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(round_up(bytes, 1 << 20), 1 << 20);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        return fail(GPZ_E_OOM, "hipHostMalloc(%zu) failed", want);
+    }
+    cap = want;
+    return GPZ_OK;
+}
+HostBuf::~HostBuf() {
+    if (p) hipHostFree(p);
+}
+
+// ---------------------------------------------------------------------------
+// plan: chunk list -> members -> segments
+
+int build_plan(Plan &pl, const uint64_t *lens, const uint64_t *dev_off, size_t C, uint32_t sub_chunk,
+               int gzip) {
+    pl = Plan();
+    if (sub_chunk == 0) return fail(GPZ_E_INVAL, "sub_chunk must be > 0");
+    pl.C = (uint32_t)C;
+    pl.chunk_dev_off.resize(C);
+    pl.chunk_mem0.resize(C + 1);
+    uint64_t in_off = 0, pos = 0, out = 0;
+    for (size_t c = 0; c < C; c++) {
+        uint64_t len = lens[c];
+        if (dev_off) {
+            if (dev_off[c] % 16) return fail(GPZ_E_INVAL, "chunk %zu offset not 16-byte aligned", c);
+            pl.chunk_dev_off[c] = dev_off[c];
+        } else {
+            pl.chunk_dev_off[c] = in_off;
+            in_off += round_up(len, IN_ALIGN);
+        }
+        pl.in_end = std::max<uint64_t>(pl.in_end, pl.chunk_dev_off[c] + len);
+        pl.chunk_mem0[c] = (uint32_t)pl.mem.size();
+        uint64_t nm = len <= sub_chunk ? 1 : (len + sub_chunk - 1) / sub_chunk;
+        for (uint64_t i = 0; i < nm; i++) {
+            MemberDesc d{};
+            uint64_t off = i * sub_chunk;
+            uint64_t n = std::min<uint64_t>(sub_chunk, len - off);
+            if (len <= sub_chunk) n = len;
+            if (n > (16u << 20)) return fail(GPZ_E_INVAL, "member of %llu bytes exceeds 16 MiB", (unsigned long long)n);
+            d.in_off = pl.chunk_dev_off[c] + off;
+            d.pos_base = pos;
+            d.n = (uint32_t)n;
+            d.nseg = (uint32_t)((n + SEG - 1) >> SEG_LOG2);
+            d.seg_base = (uint32_t)pl.seg_mem.size();
+            d.chunk = (uint32_t)c;
+            for (uint32_t s = 0; s < d.nseg; s++) pl.seg_mem.push_back((uint32_t)pl.mem.size());
+            pl.mem.push_back(d);
+            pos += round_up(n, POS_ALIGN);
+            out += 2 * n + 1024 + (gzip ? 18 : 0);
+        }
+    }
+    pl.chunk_mem0[C] = (uint32_t)pl.mem.size();
+    pl.M = (uint32_t)pl.mem.size();
+    pl.G = (uint32_t)pl.seg_mem.size();
+    pl.P = pos + POS_ALIGN;
+    pl.out_cap = round_up(out, 256) + 256;
+    pl.in_bytes = round_up(pl.in_end, IN_ALIGN) + IN_ALIGN;
+    pl.gzip = gzip;
+    return GPZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// workspace laid out by capacity
+
+static uint64_t carve(uint64_t &cur, uint64_t bytes) {
+    uint64_t o = round_up(cur, 256);
+    cur = o + round_up(bytes, 256);
+    return o;
+}
+
+int Workspace::reserve(const Caps &want) {
+    Caps c = caps;
+    bool grow = !ws.p || want.P > c.P || want.G > c.G || want.M > c.M || want.C > c.C || want.out > c.out;
+    if (!grow) return GPZ_OK;
+    c.P = std::max(c.P, want.P);
+    c.G = std::max(c.G, want.G);
+    c.M = std::max(c.M, want.M);
+    c.C = std::max(c.C, want.C);
+    c.out = std::max(c.out, want.out);
+    uint64_t cur = 0;
+    Layout L;
+    L.mem = carve(cur, c.M * sizeof(MemberDesc));
+    L.seg_mem = carve(cur, c.G * 4);
+    L.chunk_mem0 = carve(cur, (c.C + 1) * 4);
+    L.counts = carve(cur, sizeof(Counts));
+    L.desc_end = cur;
+    L.tok = carve(cur, c.P * 4);
+    L.wsum = carve(cur, (uint64_t)c.G * SPAN * 4);
+    L.seg_crc = carve(cur, c.G * 4);
+    L.seg_entry = carve(cur, c.G * 4);
+    L.seg_tokbase = carve(cur, c.G * 4);
+    L.walked = carve(cur, c.P * 4);
+    L.hist = carve(cur, (uint64_t)c.M * NSYM * 4);
+    L.ntok = carve(cur, c.M * 4);
+    L.codes = carve(cur, (uint64_t)c.M * NSYM * 4);
+    L.prefix = carve(cur, (uint64_t)c.M * PREFIX_WORDS * 4);
+    L.prefix_bits = carve(cur, c.M * 4);
+    L.mem_bytes = carve(cur, c.M * 8);
+    L.mem_crc = carve(cur, c.M * 4);
+    L.mem_off = carve(cur, c.M * 8);
+    L.chunk_off = carve(cur, c.C * 8);
+    L.chunk_len = carve(cur, c.C * 8);
+    L.total = carve(cur, 8);
+    L.out = carve(cur, c.out);
+    L.end = cur;
+    int rc = ws.ensure(L.end);
+    if (rc) return rc;
+    rc = hdesc.ensure(L.desc_end);
+    if (rc) return rc;
+    caps = c;
+    lay = L;
+    generation++;
+    uint8_t *b = (uint8_t *)ws.p;
+    Work &w = work;
+    w = Work{};
+    w.mem = (const MemberDesc *)(b + L.mem);
+    w.seg_mem = (const uint32_t *)(b + L.seg_mem);
+    w.chunk_mem0 = (const uint32_t *)(b + L.chunk_mem0);
+    w.counts = (const Counts *)(b + L.counts);
+    w.tok = (uint32_t *)(b + L.tok);
+    w.wsum = (uint32_t *)(b + L.wsum);
+    w.seg_crc = (uint32_t *)(b + L.seg_crc);
+    w.seg_entry = (uint32_t *)(b + L.seg_entry);
+    w.seg_tokbase = (uint32_t *)(b + L.seg_tokbase);
+    w.walked = (uint32_t *)(b + L.walked);
+    w.hist = (uint32_t *)(b + L.hist);
+    w.ntok = (uint32_t *)(b + L.ntok);
+    w.codes = (uint32_t *)(b + L.codes);
+    w.prefix = (uint32_t *)(b + L.prefix);
+    w.prefix_bits = (uint32_t *)(b + L.prefix_bits);
+    w.mem_bytes = (uint64_t *)(b + L.mem_bytes);
+    w.mem_crc = (uint32_t *)(b + L.mem_crc);
+    w.mem_off = (uint64_t *)(b + L.mem_off);
+    w.chunk_off = (uint64_t *)(b + L.chunk_off);
+    w.chunk_len = (uint64_t *)(b + L.chunk_len);
+    w.total = (uint64_t *)(b + L.total);
+    w.out = b + L.out;
+    w.window = 32768;
+    return GPZ_OK;
+}
+
+// Copy the plan's descriptor tables into pinned staging and enqueue the upload.
+int Workspace::upload_plan(const Plan &pl, hipStream_t st) {
+    uint8_t *h = (uint8_t *)hdesc.p;
+    memcpy(h + lay.mem, pl.mem.data(), pl.mem.size() * sizeof(MemberDesc));
+    memcpy(h + lay.seg_mem, pl.seg_mem.data(), pl.seg_mem.size() * 4);
+    memcpy(h + lay.chunk_mem0, pl.chunk_mem0.data(), pl.chunk_mem0.size() * 4);
+    Counts cn{pl.G, pl.M, pl.C, 0};
+    memcpy(h + lay.counts, &cn, sizeof(cn));
+    HIPCHK(hipMemcpyAsync(ws.p, hdesc.p, lay.desc_end, hipMemcpyHostToDevice, st));
+    return GPZ_OK;
+}
+
+Grids grids_for(const Caps &c, int n_cu) {
+    Grids g;
+    g.k1 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.G, (uint64_t)n_cu * 8));
+    g.k3 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.G, (uint64_t)n_cu * 8));
+    g.chain = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 16));
+    g.huff = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 16));
+    g.emit = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 4));
+    return g;
+}
+
+// Enqueue the kernels of one run.  Events (if given) bracket the match-find
+// and emit kernels for timing.
+void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t *ev) {
+    if (ev) hipEventRecord(ev[0], st);
+    switch (mode) {
+        case MODE_MATCH:
+            launch_match_find(w, g.k1, 0, st);
+            break;
+        case MODE_WALK:
+            launch_walk_summary(w, g.k1, st);
+            launch_chain(w, g.chain, st);
+            launch_walk_emit(w, g.k3, st);
+            break;
+        case MODE_ENCODE:
+            launch_hist_tokens(w, g.huff, st);
+            launch_huffman(w, g.huff, st);
+            launch_scan(w, st);
+            if (ev) hipEventRecord(ev[3], st);
+            launch_emit(w, g.emit, st);
+            if (ev) hipEventRecord(ev[4], st);
+            break;
+        default:  // MODE_FULL
+            launch_match_find(w, g.k1, 1, st);
+            if (ev) hipEventRecord(ev[1], st);
+            launch_chain(w, g.chain, st);
+            launch_walk_emit(w, g.k3, st);
+            launch_huffman(w, g.huff, st);
+            launch_scan(w, st);
+            if (ev) hipEventRecord(ev[3], st);
+            launch_emit(w, g.emit, st);
+            if (ev) hipEventRecord(ev[4], st);
+            break;
+    }
+    if (ev) hipEventRecord(ev[2], st);
+}
+
+// ---------------------------------------------------------------------------
+// lanes
+
+Lane::~Lane() {
+    if (graph) hipGraphExecDestroy(graph);
+    for (auto &e : ev)
+        if (e) hipEventDestroy(e);
+    if (st) hipStreamDestroy(st);
+}
+
+int Lane::init(int dev) {
+    HIPCHK(hipSetDevice(dev));
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    return GPZ_OK;
+}
+
+}  // namespace gpz
+
+using namespace gpz;
+
+struct gpz_ctx {
+    int device = 0;
+    int n_cu = 256;
+    gpz_config cfg{};
+    char label[256] = {0};
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<std::unique_ptr<Lane>> lanes;
+    std::vector<Lane *> free_lanes;
+
+    Lane *acquire() {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return !free_lanes.empty(); });
+        Lane *l = free_lanes.back();
+        free_lanes.pop_back();
+        return l;
+    }
+    void release(Lane *l) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            free_lanes.push_back(l);
+        }
+        cv.notify_one();
+    }
+};
+
+namespace gpz {
+int ctx_device(gpz_ctx *c) { return c->device; }
+int ctx_cus(gpz_ctx *c) { return c->n_cu; }
+uint32_t ctx_sub_chunk(gpz_ctx *c) { return c->cfg.sub_chunk; }
+Lane *ctx_acquire(gpz_ctx *c) { return c->acquire(); }
+void ctx_release(gpz_ctx *c, Lane *l) { c->release(l); }
+
+// Run one plan on a lane: reserve, upload descriptors, (optionally) upload the
+// host input, enqueue kernels.  Input is already in lane->din when host_in is null.
+int lane_run(gpz_ctx *ctx, Lane *ln, const Plan &pl, int mode, const uint8_t *host_in,
+             const uint64_t *host_lens) {
+    HIPCHK(hipSetDevice(ctx->device));
+    Caps want{pl.P, pl.G, pl.M, pl.C, pl.out_cap};
+    int rc = ln->wsp.reserve(want);
+    if (rc) return rc;
+    rc = ln->din.ensure(pl.in_bytes);
+    if (rc) return rc;
+    if (host_in) {
+        rc = ln->hin.ensure(pl.in_bytes);
+        if (rc) return rc;
+        uint8_t *h = (uint8_t *)ln->hin.p;
+        uint64_t src = 0;
+        for (uint32_t c = 0; c < pl.C; c++) {
+            memcpy(h + pl.chunk_dev_off[c], host_in + src, host_lens[c]);
+            src += host_lens[c];
+        }
+        HIPCHK(hipMemcpyAsync(ln->din.p, ln->hin.p, pl.in_end, hipMemcpyHostToDevice, ln->st));
+    }
+    rc = ln->wsp.upload_plan(pl, ln->st);
+    if (rc) return rc;
+    Work w = ln->wsp.work;
+    w.in = (const uint8_t *)ln->din.p;
+    w.gzip = pl.gzip;
+    w.strict = ln->strict;
+    Grids g = grids_for(ln->wsp.caps, ctx->n_cu);
+    enqueue(w, g, mode, ln->st, nullptr);
+    HIPCHK(hipGetLastError());
+    return GPZ_OK;
+}
+
+// Copy one chunk's result (after the run completed its kernels) into out.
+int lane_fetch_chunk(Lane *ln, uint32_t c, uint8_t *out, size_t out_cap, size_t *out_len) {
+    Workspace &W = ln->wsp;
+    uint64_t meta[2];
+    HIPCHK(hipMemcpyAsync(meta, W.work.chunk_off + c, 8, hipMemcpyDeviceToHost, ln->st));
+    HIPCHK(hipMemcpyAsync(meta + 1, W.work.chunk_len + c, 8, hipMemcpyDeviceToHost, ln->st));
+    HIPCHK(hipStreamSynchronize(ln->st));
+    *out_len = meta[1];
+    if (meta[1] > out_cap) return fail(GPZ_E_NOSPACE, "output needs %llu bytes", (unsigned long long)meta[1]);
+    int rc = ln->hout.ensure(meta[1] + 8);
+    if (rc) return rc;
+    if (meta[1]) {
+        HIPCHK(hipMemcpyAsync(ln->hout.p, W.work.out + meta[0], meta[1], hipMemcpyDeviceToHost, ln->st));
+        HIPCHK(hipStreamSynchronize(ln->st));
+        memcpy(out, ln->hout.p, meta[1]);
+    }
+    return GPZ_OK;
+}
+
+}  // namespace gpz
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+void gpz_config_default(gpz_config *cfg) {
+    if (!cfg) return;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->device = -1;
+    cfg->sub_chunk = 128 * 1024;
+    cfg->window = 32 * 1024;
+    cfg->lanes = 4;
+    cfg->max_in_flight = 16;
+    cfg->lane_bytes = 2u << 20;
+}
+
+const char *gpz_last_error(void) { return gpz::last_error(); }
+
+const char *gpz_status_string(int s) {
+    switch (s) {
+        case GPZ_OK: return "ok";
+        case GPZ_E_NODEV: return "no usable GPU";
+        case GPZ_E_OOM: return "out of memory";
+        case GPZ_E_INVAL: return "invalid argument";
+        case GPZ_E_DEVICE: return "device error";
+        case GPZ_E_NOSPACE: return "output buffer too small";
+        case GPZ_E_BUSY: return "admission refused";
+        case GPZ_E_AGAIN: return "nothing ready";
+        default: return "unknown status";
+    }
+}
+
+int gpz_device_count(int *count) {
+    if (!count) return fail(GPZ_E_INVAL, "count is null");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return GPZ_OK;
+}
+
+int gpz_init(const gpz_config *cfg_in, gpz_ctx **out) {
+    clear_error();
+    if (!out) return fail(GPZ_E_INVAL, "out is null");
+    *out = nullptr;
+    gpz_config cfg;
+    if (cfg_in)
+        cfg = *cfg_in;
+    else
+        gpz_config_default(&cfg);
+    if (cfg.sub_chunk == 0 || cfg.sub_chunk > (16u << 20)) return fail(GPZ_E_INVAL, "sub_chunk out of range");
+    if (cfg.window != 32768) return fail(GPZ_E_INVAL, "only the 32 KiB DEFLATE window is supported");
+    if (cfg.lanes == 0) cfg.lanes = 1;
+    if (cfg.lanes > 32) cfg.lanes = 32;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(GPZ_E_NODEV, "no HIP device");
+    int dev = cfg.device;
+    if (dev < 0) {
+        // GPZIP_GPU index semantics of the reference (codec-gpu/src/gpu/context.rs:119-155);
+        // "high"/"low" pick device 0 (one GPU per process here).
+        const char *e = getenv("GPZIP_GPU");
+        dev = 0;
+        if (e && *e && strcmp(e, "high") && strcmp(e, "low")) {
+            char *end = nullptr;
+            long v = strtol(e, &end, 10);
+            if (end && *end == 0 && v >= 0) dev = (int)v;
+        }
+    }
+    if (dev >= n) return fail(GPZ_E_NODEV, "GPZIP_GPU=%d but only %d device(s) available", dev, n);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return fail(GPZ_E_NODEV, "cannot query device %d", dev);
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(GPZ_E_NODEV, "device %d is %s; this library is built for gfx950", dev, prop.gcnArchName);
+    if (hipSetDevice(dev) != hipSuccess) return fail(GPZ_E_NODEV, "hipSetDevice(%d) failed", dev);
+    if (gpz::upload_constants() != hipSuccess) return fail(GPZ_E_DEVICE, "constant upload failed");
+    auto ctx = std::make_unique<gpz_ctx>();
+    ctx->device = dev;
+    ctx->n_cu = prop.multiProcessorCount;
+    ctx->cfg = cfg;
+    snprintf(ctx->label, sizeof(ctx->label), "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    for (uint32_t i = 0; i < cfg.lanes; i++) {
+        auto ln = std::make_unique<Lane>();
+        int rc = ln->init(dev);
+        if (rc) return rc;
+        ctx->free_lanes.push_back(ln.get());
+        ctx->lanes.push_back(std::move(ln));
+    }
+    *out = ctx.release();
+    return GPZ_OK;
+}
+
+void gpz_destroy(gpz_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipDeviceSynchronize();
+    delete ctx;
+}
+
+int gpz_device_label(gpz_ctx *ctx, char *buf, size_t cap) {
+    if (!ctx || !buf || !cap) return fail(GPZ_E_INVAL, "bad argument");
+    snprintf(buf, cap, "%s", ctx->label);
+    return GPZ_OK;
+}
+
+size_t gpz_compress_bound(const gpz_ctx *ctx, size_t len) {
+    uint64_t sub = ctx ? ctx->cfg.sub_chunk : 131072;
+    uint64_t members = len <= sub ? 1 : (len + sub - 1) / sub;
+    return 2 * len + members * (1024 + 18);
+}
+
+int gpz_compress_chunk(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out, size_t out_cap,
+                       size_t *out_len) {
+    clear_error();
+    if (!ctx || (!in && len) || !out_len) return fail(GPZ_E_INVAL, "bad argument");
+    Plan pl;
+    uint64_t l = len;
+    int rc = build_plan(pl, &l, nullptr, 1, ctx->cfg.sub_chunk, 1);
+    if (rc) return rc;
+    Lane *ln = ctx->acquire();
+    ln->strict = 0;
+    rc = lane_run(ctx, ln, pl, MODE_FULL, in, &l);
+    if (!rc) rc = lane_fetch_chunk(ln, 0, out, out_cap, out_len);
+    if (rc) hipStreamSynchronize(ln->st);
+    ctx->release(ln);
+    return rc;
+}
+
+int gpz_gzip_member(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out, size_t out_cap,
+                    size_t *out_len) {
+    clear_error();
+    if (!ctx || (!in && len) || !out_len) return fail(GPZ_E_INVAL, "bad argument");
+    if (len > (16u << 20)) return fail(GPZ_E_INVAL, "member larger than 16 MiB");
+    Plan pl;
+    uint64_t l = len;
+    int rc = build_plan(pl, &l, nullptr, 1, (uint32_t)std::max<size_t>(len, 1), 1);
+    if (rc) return rc;
+    Lane *ln = ctx->acquire();
+    ln->strict = 0;
+    rc = lane_run(ctx, ln, pl, MODE_FULL, in, &l);
+    if (!rc) rc = lane_fetch_chunk(ln, 0, out, out_cap, out_len);
+    if (rc) hipStreamSynchronize(ln->st);
+    ctx->release(ln);
+    return rc;
+}
+
+int gpz_match_find(gpz_ctx *ctx, const uint8_t *in, size_t len, uint32_t *tokens) {
+    clear_error();
+    if (!ctx || (!in && len) || (!tokens && len)) return fail(GPZ_E_INVAL, "bad argument");
+    if (len == 0) return GPZ_OK;
+    if (len > (16u << 20)) return fail(GPZ_E_INVAL, "member larger than 16 MiB");
+    Plan pl;
+    uint64_t l = len;
+    int rc = build_plan(pl, &l, nullptr, 1, (uint32_t)len, 1);
+    if (rc) return rc;
+    Lane *ln = ctx->acquire();
+    ln->strict = 0;
+    rc = lane_run(ctx, ln, pl, MODE_MATCH, in, &l);
+    if (!rc) {
+        int r2 = ln->hout.ensure(len * 4);
+        if (r2) {
+            rc = r2;
+        } else if (hipMemcpyAsync(ln->hout.p, ln->wsp.work.tok, len * 4, hipMemcpyDeviceToHost, ln->st) != hipSuccess ||
+                   hipStreamSynchronize(ln->st) != hipSuccess) {
+            rc = fail(GPZ_E_DEVICE, "token read-back failed");
+        } else {
+            memcpy(tokens, ln->hout.p, len * 4);
+        }
+    }
+    if (rc) hipStreamSynchronize(ln->st);
+    ctx->release(ln);
+    return rc;
+}
+
+int gpz_walk(gpz_ctx *ctx, const uint8_t *in, const uint32_t *per_pos, size_t len, int strict,
+             uint32_t *walked, size_t *n_walked) {
+    clear_error();
+    if (!ctx || !n_walked || (len && (!per_pos || !walked || (!strict && !in)))) return fail(GPZ_E_INVAL, "bad argument");
+    *n_walked = 0;
+    if (len == 0) return GPZ_OK;
+    if (len > (16u << 20)) return fail(GPZ_E_INVAL, "more than 16 Mi positions");
+    for (size_t i = 0; i < len; i++)
+        if ((per_pos[i] >> 16) > 258) return fail(GPZ_E_INVAL, "token %zu: length %u > 258", i, per_pos[i] >> 16);
+    Plan pl;
+    uint64_t l = len;
+    int rc = build_plan(pl, &l, nullptr, 1, (uint32_t)len, 1);
+    if (rc) return rc;
+    Lane *ln = ctx->acquire();
+    ln->strict = strict ? 1 : 0;
+    do {
+        Caps want{pl.P, pl.G, pl.M, pl.C, pl.out_cap};
+        if ((rc = ln->wsp.reserve(want))) break;
+        if ((rc = ln->din.ensure(pl.in_bytes))) break;
+        if ((rc = ln->hin.ensure(std::max<uint64_t>(pl.in_bytes, len * 4)))) break;
+        if (hipSetDevice(ctx->device) != hipSuccess) {
+            rc = fail(GPZ_E_DEVICE, "hipSetDevice failed");
+            break;
+        }
+        // tokens -> w.tok, input bytes -> din
+        memcpy(ln->hin.p, per_pos, len * 4);
+        if (hipMemcpyAsync(ln->wsp.work.tok, ln->hin.p, len * 4, hipMemcpyHostToDevice, ln->st) != hipSuccess ||
+            hipStreamSynchronize(ln->st) != hipSuccess) {
+            rc = fail(GPZ_E_DEVICE, "token upload failed");
+            break;
+        }
+        if (in) {
+            memcpy(ln->hin.p, in, len);
+            if (hipMemcpyAsync(ln->din.p, ln->hin.p, len, hipMemcpyHostToDevice, ln->st) != hipSuccess) {
+                rc = fail(GPZ_E_DEVICE, "input upload failed");
+                break;
+            }
+        }
+        if ((rc = ln->wsp.upload_plan(pl, ln->st))) break;
+        Work w = ln->wsp.work;
+        w.in = (const uint8_t *)ln->din.p;
+        w.strict = ln->strict;
+        w.gzip = 1;
+        enqueue(w, grids_for(ln->wsp.caps, ctx->n_cu), MODE_WALK, ln->st, nullptr);
+        uint32_t nt = 0;
+        if (hipMemcpyAsync(&nt, w.ntok, 4, hipMemcpyDeviceToHost, ln->st) != hipSuccess ||
+            hipStreamSynchronize(ln->st) != hipSuccess) {
+            rc = fail(GPZ_E_DEVICE, "walk failed");
+            break;
+        }
+        if ((rc = ln->hout.ensure((uint64_t)nt * 4 + 4))) break;
+        if (nt && (hipMemcpyAsync(ln->hout.p, w.walked, (uint64_t)nt * 4, hipMemcpyDeviceToHost, ln->st) != hipSuccess ||
+                   hipStreamSynchronize(ln->st) != hipSuccess)) {
+            rc = fail(GPZ_E_DEVICE, "walk read-back failed");
+            break;
+        }
+        memcpy(walked, ln->hout.p, (uint64_t)nt * 4);
+        *n_walked = nt;
+    } while (0);
+    if (rc) hipStreamSynchronize(ln->st);
+    ln->strict = 0;
+    ctx->release(ln);
+    return rc;
+}
+
+int gpz_encode_block(gpz_ctx *ctx, const uint32_t *walked, size_t n, uint8_t *out, size_t out_cap,
+                     size_t *out_len) {
+    clear_error();
+    if (!ctx || !out_len || (n && !walked)) return fail(GPZ_E_INVAL, "bad argument");
+    if (n > (16u << 20)) return fail(GPZ_E_INVAL, "more than 16 Mi tokens");
+    for (size_t i = 0; i < n; i++) {
+        uint32_t len = walked[i] >> 16, v = walked[i] & 0xFFFFu;
+        bool ok = len == 0 ? v <= 255 : (len >= 3 && len <= 258 && v >= 1 && v <= 32768);
+        if (!ok) return fail(GPZ_E_INVAL, "token %zu outside the DEFLATE alphabet", i);
+    }
+    Plan pl;
+    uint64_t l = n;
+    int rc = build_plan(pl, &l, nullptr, 1, (uint32_t)std::max<size_t>(n, 1), 0);
+    if (rc) return rc;
+    Lane *ln = ctx->acquire();
+    do {
+        Caps want{pl.P, pl.G, pl.M, pl.C, pl.out_cap};
+        if ((rc = ln->wsp.reserve(want))) break;
+        if ((rc = ln->din.ensure(pl.in_bytes))) break;
+        if ((rc = ln->hin.ensure(n * 4 + 64))) break;
+        if (hipSetDevice(ctx->device) != hipSuccess) {
+            rc = fail(GPZ_E_DEVICE, "hipSetDevice failed");
+            break;
+        }
+        memcpy(ln->hin.p, walked, n * 4);
+        uint32_t nt = (uint32_t)n;
+        Work w = ln->wsp.work;
+        if ((n && hipMemcpyAsync(w.walked, ln->hin.p, n * 4, hipMemcpyHostToDevice, ln->st) != hipSuccess) ||
+            hipMemcpyAsync(w.ntok, &nt, 4, hipMemcpyHostToDevice, ln->st) != hipSuccess ||
+            hipStreamSynchronize(ln->st) != hipSuccess) {
+            rc = fail(GPZ_E_DEVICE, "token upload failed");
+            break;
+        }
+        if ((rc = ln->wsp.upload_plan(pl, ln->st))) break;
+        w.in = (const uint8_t *)ln->din.p;
+        w.gzip = 0;
+        enqueue(w, grids_for(ln->wsp.caps, ctx->n_cu), MODE_ENCODE, ln->st, nullptr);
+        rc = lane_fetch_chunk(ln, 0, out, out_cap, out_len);
+    } while (0);
+    if (rc) hipStreamSynchronize(ln->st);
+    ctx->release(ln);
+    return rc;
+}
+
+}  // extern "C"

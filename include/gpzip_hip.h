@@ -1,0 +1,204 @@
+/*
+ * gpzip_hip.h -- C ABI of libgpzip_hip.so, the MI355X (gfx950) gzip compress
+ * path that stands in for gpzip's GPU backend.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never
+ * unwinds, and reports an int status (GPZ_OK or a GPZ_E_* code).  A
+ * human-readable message for the most recent failure on the calling thread
+ * is available from gpz_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the reference repo,
+ * crates/gpzip- prefix dropped):
+ *   gpz_init / gpz_destroy    GpuBackend::try_init + GpuContext::try_init
+ *                             (codec-gpu/src/gpu.rs:125-186,
+ *                              codec-gpu/src/gpu/context.rs:65-106)
+ *   gpz_compress_chunk        GpuBackend::gzip_chunk_fn closure
+ *                             (codec-gpu/src/gpu.rs:193-208)
+ *   gpz_gzip_member           encode_one (codec-gpu/src/gpu.rs:271-287)
+ *   gpz_match_find            BatchedLz77::submit / Lz77HashPipeline::match_find
+ *                             (codec-gpu/src/gpu/batch.rs:74-85,
+ *                              codec-gpu/src/gpu/lz77_hash.rs:306-453)
+ *   gpz_walk                  greedy_walk (codec-gpu/src/gpu/lz77.rs:260-284);
+ *                             strict: BlockWalkPipeline::walk /
+ *                             host_strict_greedy (codec-gpu/src/gpu/walk.rs:291-462, 512-525)
+ *   gpz_encode_block          encode_block_fast (codec-gpu/src/gpu/deflate.rs:752-959)
+ *   gpz_batch_*               BatchedLz77 submitter/completer pair
+ *                             (codec-gpu/src/gpu/batch.rs:48-141) -- here a whole
+ *                             batch of uneven chunks runs as one graph launch
+ *   gpz_plan_*                device-resident form of gpz_batch_* (inputs and
+ *                             outputs already in HBM); no reference counterpart
+ *                             beyond Lz77HashPipeline::submit_batch_async/collect_async
+ *                             (codec-gpu/src/gpu/lz77_hash.rs:473-664)
+ *   gpz_server_*              no reference counterpart (serving form); its
+ *                             in-order member stream follows ParallelChunkedWriter's
+ *                             ordering contract (codec-cpu/src/parallel.rs:144-150)
+ *
+ * Token format (reference lz77_hash.rs:50-62): one uint32 per token,
+ * length << 16 | (distance or literal byte); length == 0 => literal.
+ */
+#ifndef GPZIP_HIP_H
+#define GPZIP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GPZ_OK = 0,
+    GPZ_E_NODEV = 1,   /* no usable GPU / bad device index */
+    GPZ_E_OOM = 2,     /* device or pinned-host allocation failed */
+    GPZ_E_INVAL = 3,   /* bad argument or token outside the DEFLATE alphabet */
+    GPZ_E_DEVICE = 4,  /* a HIP call failed */
+    GPZ_E_NOSPACE = 5, /* caller's output buffer too small; *out_len holds the need */
+    GPZ_E_BUSY = 6,    /* serving form: admission refused (budget) */
+    GPZ_E_AGAIN = 7    /* serving form: nothing ready yet */
+};
+
+typedef struct gpz_ctx gpz_ctx;
+
+typedef struct gpz_config {
+    int device;            /* HIP device index; -1 = take GPZIP_GPU env or 0 */
+    uint32_t sub_chunk;    /* member size; reference fixes 128 KiB (gpu.rs:160) */
+    uint32_t window;       /* DEFLATE window; reference fixes 32 KiB (lz77_hash.rs:48) */
+    uint32_t lanes;        /* concurrent per-chunk pipelines (streams) */
+    uint32_t max_in_flight;/* advertised to ParallelChunkedWriter (gpu.rs:165) */
+    uint32_t flags;        /* reserved, 0 */
+    uint64_t lane_bytes;   /* per-lane input capacity reserved at init (0 = 2 MiB) */
+} gpz_config;
+
+/* Fill *cfg with the reference's defaults: 128 KiB members, 32 KiB window,
+ * 16 in flight. */
+void gpz_config_default(gpz_config *cfg);
+
+/* Select the device, check it is gfx950, load the kernels and build the
+ * lane pool.  GPZ_E_NODEV when no GPU is present (the caller then falls back
+ * to its CPU path, as LazyGpuBackend::try_get does at gpu.rs:53-57). */
+int gpz_init(const gpz_config *cfg, gpz_ctx **out);
+void gpz_destroy(gpz_ctx *ctx);
+
+/* Message of the most recent failure on this thread ("" if none). */
+const char *gpz_last_error(void);
+const char *gpz_status_string(int status);
+
+/* Number of HIP devices visible (0 on a host without a GPU). */
+int gpz_device_count(int *count);
+
+/* Device label, e.g. "AMD Instinct MI355X (gfx950)". */
+int gpz_device_label(gpz_ctx *ctx, char *buf, size_t cap);
+
+/* Upper bound of compressed bytes for a chunk of len bytes. */
+size_t gpz_compress_bound(const gpz_ctx *ctx, size_t len);
+
+/* ---- plain per-chunk form --------------------------------------------- */
+
+/* Compress one chunk into ceil(len / sub_chunk) concatenated gzip members
+ * (one member when len <= sub_chunk, including len == 0).  Thread-safe and
+ * blocking; concurrent callers run on separate lanes (streams).  On
+ * GPZ_E_NOSPACE *out_len is set to the required size. */
+int gpz_compress_chunk(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out, size_t out_cap,
+                       size_t *out_len);
+
+/* ---- stage-level entry points (parity / debugging) --------------------- */
+
+/* Per-position packed tokens for one member (len <= 16 MiB). */
+int gpz_match_find(gpz_ctx *ctx, const uint8_t *in, size_t len, uint32_t *tokens);
+
+/* Walk per-position tokens into the emitted token stream.  strict = 0: lazy
+ * rule of greedy_walk (needs in for demoted literals); strict = 1: the
+ * strict rule of host_strict_greedy (in may be NULL).  Token lengths must be
+ * 0..258.  walked must hold len entries; *n_walked receives the count. */
+int gpz_walk(gpz_ctx *ctx, const uint8_t *in, const uint32_t *per_pos, size_t len, int strict,
+             uint32_t *walked, size_t *n_walked);
+
+/* Raw DEFLATE (one BFINAL dynamic block) for a walked token stream. */
+int gpz_encode_block(gpz_ctx *ctx, const uint32_t *walked, size_t n, uint8_t *out, size_t out_cap,
+                     size_t *out_len);
+
+/* One gzip member for len bytes (any len <= 16 MiB, no sub-chunk split). */
+int gpz_gzip_member(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out, size_t out_cap,
+                    size_t *out_len);
+
+/* ---- batched form (host buffers) ---------------------------------------- */
+
+typedef struct gpz_desc {
+    uint64_t offset; /* byte offset of the chunk in base */
+    uint64_t len;    /* chunk length */
+} gpz_desc;
+
+typedef struct gpz_batch gpz_batch;
+
+/* Launch all chunks as one batch (one graph replay on one lane).  Returns
+ * as soon as the work is enqueued; inputs are copied before return. */
+int gpz_batch_submit(gpz_ctx *ctx, const uint8_t *base, const gpz_desc *descs, size_t n,
+                     gpz_batch **out);
+int gpz_batch_wait(gpz_batch *b);
+/* Result of chunk i: pointer valid until gpz_batch_free. */
+int gpz_batch_result(gpz_batch *b, size_t i, const uint8_t **data, size_t *len);
+void gpz_batch_free(gpz_batch *b);
+
+/* ---- device-resident form (inputs and outputs in HBM) ------------------- */
+
+typedef struct gpz_plan gpz_plan;
+
+enum { GPZ_PLAN_GRAPH = 1u, GPZ_PLAN_PERSISTENT = 2u };
+
+typedef struct gpz_tune {
+    uint32_t k1_grid;    /* persistent grid of the match-find kernel (0 = one WG per segment) */
+    uint32_t k3_grid;    /* persistent grid of the walk/compact kernel (0 = one WG per segment) */
+    uint32_t emit_grid;  /* persistent grid of the emit kernel (0 = one WG per member) */
+    uint32_t reserved;
+} gpz_tune;
+
+/* Plan for a fixed list of chunks laid out in a device buffer: workspace is
+ * allocated once; gpz_plan_run may then be called repeatedly.  Chunk offsets
+ * must be multiples of 16.  d_in must have 256 readable bytes past the last
+ * chunk. */
+int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flags,
+                    const gpz_tune *tune, gpz_plan **out);
+/* Output capacity the plan needs in d_out. */
+size_t gpz_plan_out_capacity(const gpz_plan *p);
+/* Enqueue one run on the plan's stream (graph replay with GPZ_PLAN_GRAPH). */
+int gpz_plan_run(gpz_plan *p, const void *d_in, void *d_out);
+int gpz_plan_sync(gpz_plan *p);
+/* After sync: per-chunk output offset/length in d_out and the total. */
+int gpz_plan_result(gpz_plan *p, uint64_t *chunk_off, uint64_t *chunk_len, uint64_t *total);
+/* Average device time (ms) of the last run's kernels, from HIP events on
+ * the plan's stream: [0] whole run, [1] match-find kernel, [2] emit kernel. */
+int gpz_plan_timing(gpz_plan *p, float *ms, size_t n);
+void gpz_plan_destroy(gpz_plan *p);
+
+/* ---- serving form -------------------------------------------------------- */
+
+typedef struct gpz_server gpz_server;
+
+typedef struct gpz_server_config {
+    uint64_t budget_bytes;     /* device-memory budget for admitted work */
+    uint32_t max_batch_chunks; /* chunks gathered into one device step */
+    uint32_t max_batch_bytes;  /* input bytes gathered into one device step */
+    uint32_t chunk_bytes;      /* request split size (2 MiB) */
+    uint32_t streams;          /* device steps in flight */
+} gpz_server_config;
+
+void gpz_server_config_default(gpz_server_config *cfg);
+int gpz_server_start(gpz_ctx *ctx, const gpz_server_config *cfg, gpz_server **out);
+/* Admit a request (the bytes are copied).  GPZ_E_BUSY if it can never fit
+ * the budget; otherwise blocks until admitted. */
+int gpz_request_submit(gpz_server *s, const uint8_t *in, size_t len, uint64_t tag);
+/* Next finished request, in completion order; each request's members are
+ * concatenated in input order.  timeout_ms < 0 waits forever.  The buffer is
+ * owned by the server until gpz_request_release(tag). */
+int gpz_request_next(gpz_server *s, int timeout_ms, uint64_t *tag, const uint8_t **data,
+                     size_t *len, double *latency_s);
+int gpz_request_release(gpz_server *s, uint64_t tag);
+/* Counters: [0] requests done, [1] input bytes done, [2] device steps,
+ * [3] peak admitted bytes. */
+int gpz_server_stats(gpz_server *s, uint64_t *out, size_t n);
+void gpz_server_stop(gpz_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

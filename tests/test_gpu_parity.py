@@ -1,0 +1,199 @@
+"""HIP path vs the CPU oracle, bit-exact, through the C ABI (needs a gfx950 GPU)."""
+import base64
+import gzip
+import json
+import os
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+import gpzip_amd
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def pack(pairs):
+    return np.array([(l << 16) | v for l, v in pairs], dtype=np.uint32)
+
+
+def first_diff(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    n = min(a.size, b.size)
+    d = np.nonzero(a[:n] != b[:n])[0]
+    return int(d[0]) if d.size else (n if a.size != b.size else -1)
+
+
+def assert_tokens_equal(got, exp, what):
+    i = first_diff(got, exp)
+    assert i < 0, f"{what}: first difference at {i}: got {int(got[i]) if i < got.size else None:#x} " \
+                  f"expected {int(exp[i]) if i < exp.size else None:#x}"
+
+
+@pytest.mark.parametrize("kat", GOLD["match_kats"], ids=lambda k: k["name"])
+def test_match_find_known_answers(ctx, kat):
+    data = base64.b64decode(kat["input_b64"])
+    say("match_find KAT", kat["name"], len(data))
+    t = ctx.match_find(data)
+    assert_tokens_equal(t, O.match_find(data), kat["name"])
+    for p, (l, v) in kat["expect"].items():
+        assert (int(t[int(p)]) >> 16, int(t[int(p)]) & 0xFFFF) == (l, v)
+
+
+SIZES = [1, 2, 3, 4, 5, 63, 4095, 4096, 4097, 8191, 12289, 40000, 131071, 131072]
+
+
+@pytest.mark.parametrize("kind", ["text", "binmix", "log", "random", "zeros"])
+@pytest.mark.parametrize("n", SIZES)
+def test_match_find_equals_oracle(ctx, kind, n):
+    if kind == "random":
+        data = np.random.default_rng(n).integers(0, 256, n, dtype=np.uint8)
+    elif kind == "zeros":
+        data = np.zeros(n, np.uint8)
+    else:
+        data = gpzip_amd.synth(kind, n, 100 + n)
+    say("match_find", kind, n)
+    assert_tokens_equal(ctx.match_find(data), O.match_find(data), f"{kind} {n}")
+
+
+@pytest.mark.parametrize("case", GOLD["walk_strict"], ids=lambda c: c["name"])
+def test_strict_walk_reference_cases(ctx, case):
+    if "all_literals" in case:
+        toks = pack([(0, i & 0xFF) for i in range(case["all_literals"])])
+        exp = toks
+    else:
+        toks, exp = pack(case["tokens"]), pack(case["expect"])
+    say("strict walk", case["name"], toks.size)
+    assert_tokens_equal(ctx.walk(toks, strict=True), exp, case["name"])
+
+
+@pytest.mark.parametrize("case", GOLD["walk_lazy"], ids=lambda c: c["name"])
+def test_lazy_walk_cases(ctx, case):
+    data = base64.b64decode(case["data_b64"])
+    say("lazy walk", case["name"])
+    assert_tokens_equal(ctx.walk(pack(case["tokens"]), data), pack(case["expect"]), case["name"])
+
+
+@pytest.mark.parametrize("kind", ["text", "binmix", "log"])
+@pytest.mark.parametrize("n", [4096, 4097, 70000, 131072])
+def test_walk_equals_oracle(ctx, kind, n):
+    data = gpzip_amd.synth(kind, n, 7 + n)
+    t = O.match_find(data)
+    say("walk", kind, n)
+    assert_tokens_equal(ctx.walk(t, data), O.greedy_walk(t, data), f"lazy {kind} {n}")
+    assert_tokens_equal(ctx.walk(t, strict=True), O.strict_walk(t), f"strict {kind} {n}")
+
+
+@pytest.mark.parametrize("kind", ["text", "binmix", "log", "random"])
+def test_encode_block_equals_oracle(ctx, kind):
+    n = 131072
+    data = (np.random.default_rng(5).integers(0, 256, n, dtype=np.uint8) if kind == "random"
+            else gpzip_amd.synth(kind, n, 21))
+    w = O.greedy_walk(O.match_find(data), data)
+    say("encode_block", kind, w.size)
+    got = ctx.encode_block(w)
+    exp = O.encode_block(w)
+    assert got == exp, f"first byte difference at {first_diff(np.frombuffer(got, np.uint8), np.frombuffer(exp, np.uint8))}"
+    assert zlib.decompress(got, -15) == data.tobytes()
+
+
+def test_encode_block_small_cases(ctx):
+    say("encode_block small")
+    assert ctx.encode_block(np.zeros(0, np.uint32)).hex() == GOLD["encode_empty_hex"]
+    for toks in ([(0, 97)], [(0, 0), (0, 255)], [(0, 120), (3, 1)], [(0, 1), (258, 1), (4, 2)]):
+        t = pack(toks)
+        assert ctx.encode_block(t) == O.encode_block(t), toks
+
+
+def test_encode_block_rejects_bad_tokens(ctx):
+    with pytest.raises(gpzip_amd.GpzError):
+        ctx.encode_block(pack([(2, 1)]))
+    with pytest.raises(gpzip_amd.GpzError):
+        ctx.encode_block(pack([(0, 300)]))
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 100, 4096, 65537, 131072])
+def test_gzip_member_equals_oracle(ctx, n):
+    data = gpzip_amd.synth("binmix", max(n, 1), 3)[:n]
+    say("gzip_member", n)
+    got = ctx.gzip_member(data)
+    assert got == O.gzip_member(data)
+    assert gzip.decompress(got) == data.tobytes()
+
+
+@pytest.mark.parametrize("kind", ["text", "binmix", "log"])
+def test_compress_chunk_2mib_equals_oracle(ctx, kind):
+    data = gpzip_amd.synth(kind, 2 << 20, {"text": 0, "binmix": 1, "log": 2}[kind])
+    say("compress_chunk", kind)
+    got = ctx.compress_chunk(data)
+    exp = O.gzip_chunk(data)
+    assert got == exp, f"first byte difference at {first_diff(np.frombuffer(got, np.uint8), np.frombuffer(exp, np.uint8))}"
+    assert gzip.decompress(got) == data.tobytes()
+
+
+@pytest.mark.parametrize("n", [0, 1, 131073, 262144, 300001])
+def test_compress_chunk_edge_sizes(ctx, n):
+    data = gpzip_amd.synth("log", max(n, 1), n)[:n]
+    say("compress_chunk edge", n)
+    got = ctx.compress_chunk(data)
+    assert got == O.gzip_chunk(data)
+    assert gzip.decompress(got) == data.tobytes()
+
+
+def test_batch_equals_plain(ctx):
+    rng = np.random.default_rng(9)
+    sizes = [0, 1, 5000, 131072, 131073, 700000, 2 << 20, 3, 4096 * 33 + 17]
+    kinds = ["text", "binmix", "log"]
+    chunks = [gpzip_amd.synth(kinds[i % 3], max(s, 1), 40 + i)[:s] for i, s in enumerate(sizes)]
+    say("batch", len(chunks))
+    res = ctx.batch(chunks)
+    for c, r in zip(chunks, res):
+        assert r == ctx.compress_chunk(c)
+        assert r == O.gzip_chunk(c)
+    # the same batch twice (graph replay) gives the same bytes
+    assert ctx.batch(chunks) == res
+
+
+def test_server_alone_equals_mix(ctx):
+    kinds = ["text", "binmix", "log"]
+    reqs = [gpzip_amd.synth(kinds[i % 3], s, 70 + i)
+            for i, s in enumerate([1000, 300000, 5 << 20, 4096, 2 << 20, 123457])]
+    say("server alone")
+    alone = {}
+    srv = gpzip_amd.Server(ctx, streams=2)
+    try:
+        for i, r in enumerate(reqs):
+            srv.submit(r, i)
+            tag, data, lat = srv.next()
+            assert tag == i
+            alone[i] = data
+        say("server mix")
+        for i, r in enumerate(reqs):
+            srv.submit(r, 100 + i)
+        mixed = {}
+        while len(mixed) < len(reqs):
+            tag, data, lat = srv.next()
+            mixed[tag - 100] = data
+    finally:
+        srv.stop()
+    for i, r in enumerate(reqs):
+        assert alone[i] == mixed[i]
+        assert alone[i] == O.gzip_chunk(r, 131072)
+        assert gzip.decompress(alone[i]) == r.tobytes()
+
+
+def test_determinism_repeat(ctx):
+    data = gpzip_amd.synth("binmix", 1 << 20, 77)
+    say("determinism")
+    a = ctx.compress_chunk(data)
+    for _ in range(3):
+        assert ctx.compress_chunk(data) == a
