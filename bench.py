@@ -70,8 +70,8 @@ class DevicePlan:
         return off, ln, tot.value
 
     def timing(self):
-        v = (C.c_float * 3)()
-        gpzip_amd._check(self.L.gpz_plan_timing(self.h, v, 3))
+        v = (C.c_float * 4)()
+        gpzip_amd._check(self.L.gpz_plan_timing(self.h, v, 4))
         return list(v)
 
     def close(self):
@@ -172,15 +172,17 @@ def plain_and_batched(torch, ctx, data: np.ndarray, sizes, steps, warmup, check_
     d_out_t = torch.zeros(tplan.out_cap + 256, dtype=torch.uint8, device="cuda:0")
     tplan.run(base_in, d_out_t.data_ptr())
     tplan.sync()
-    k1, emit, whole = [], [], []
+    k1, emit, whole, k0 = [], [], [], []
     for _ in range(max(steps, 3)):
         tplan.run(base_in, d_out_t.data_ptr())
         tplan.sync()
-        w_, a_, e_ = tplan.timing()
+        w_, a_, e_, z_ = tplan.timing()
         whole.append(w_)
         k1.append(a_)
         emit.append(e_)
+        k0.append(z_)
     res["k1_ms"] = float(np.mean(k1))
+    res["k0_ms"] = float(np.mean(k0))
     res["emit_ms"] = float(np.mean(emit))
     res["pipeline_ms"] = float(np.mean(whole))
     for p in plans:
@@ -329,7 +331,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_match_find", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
                      "traffic": None, "alg_bytes_per_launch": alg_bytes,
-                     "avg_launch_ms": round(r["k1_ms"], 4), "emit_ms": round(r["emit_ms"], 4),
+                     "avg_launch_ms": round(r["k1_ms"], 4), "seg_table_ms": round(r["k0_ms"], 4),
+                     "emit_ms": round(r["emit_ms"], 4),
                      "pipeline_ms": round(r["pipeline_ms"], 4)},
     }
     if not args.no_cpu:

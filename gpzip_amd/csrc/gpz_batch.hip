@@ -216,7 +216,7 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
     p->grids = grids_for(p->lane.wsp.caps, ctx_cus(ctx));
     if (!(flags & GPZ_PLAN_PERSISTENT)) {
         // one workgroup per segment / member
-        p->grids.k1 = p->grids.k3 = std::max<uint32_t>(1, p->pl.G);
+        p->grids.k0 = p->grids.k1 = p->grids.k3 = std::max<uint32_t>(1, p->pl.G);
         p->grids.emit = p->grids.huff = p->grids.chain = std::max<uint32_t>(1, p->pl.M);
     }
     if (tune) {
@@ -280,13 +280,14 @@ int gpz_plan_result(gpz_plan *p, uint64_t *chunk_off, uint64_t *chunk_len, uint6
 int gpz_plan_timing(gpz_plan *p, float *ms, size_t n) {
     if (!p || !ms) return fail(GPZ_E_INVAL, "bad argument");
     Lane &ln = p->lane;
-    float v[3] = {0, 0, 0};
+    float v[4] = {0, 0, 0, 0};
     HIPCHK(hipEventElapsedTime(&v[0], ln.ev[0], ln.ev[2]));
     if (p->timed) {
-        HIPCHK(hipEventElapsedTime(&v[1], ln.ev[0], ln.ev[1]));
+        HIPCHK(hipEventElapsedTime(&v[1], ln.ev[5], ln.ev[1]));
         HIPCHK(hipEventElapsedTime(&v[2], ln.ev[3], ln.ev[4]));
+        HIPCHK(hipEventElapsedTime(&v[3], ln.ev[0], ln.ev[5]));
     }
-    for (size_t i = 0; i < n && i < 3; i++) ms[i] = v[i];
+    for (size_t i = 0; i < n && i < 4; i++) ms[i] = v[i];
     return GPZ_OK;
 }
 

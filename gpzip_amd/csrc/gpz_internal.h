@@ -45,6 +45,8 @@ struct Work {
     const uint32_t *chunk_mem0; // [C] first member of each chunk
     const Counts *counts;
     uint32_t *tok;              // [P] per-position tokens
+    uint64_t *segtab;           // [G * 4096] per-segment compact (hash|first<<16, last) lists
+    uint32_t *seg_ntab;         // [G] entries in each list
     uint32_t *wsum;             // [G * SPAN] walk summary: exit | count << 16
     uint32_t *seg_crc;          // [G] raw CRC-32 (init 0, no final xor) of the segment
     uint32_t *seg_entry;        // [G] walk entry offset into the segment

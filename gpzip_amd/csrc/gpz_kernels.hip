@@ -38,7 +38,7 @@ namespace gpz {
 // constants
 
 __constant__ uint32_t c_crc_tab[256];   // reflected CRC-32 byte table
-__constant__ uint32_t c_crc_fold[256];  // x^(8*16*(255-t)) mod P, reflected
+__constant__ uint32_t c_crc_fold[512];  // x^(8*8*(511-t)) mod P, reflected
 __constant__ uint32_t c_xpow[32];       // x^(8*2^k) mod P, reflected
 
 __constant__ uint16_t c_len_base[29] = {3,  4,  5,  6,  7,  8,  9,  10, 11,  13,  15,  17,  19,  23, 27,
@@ -47,8 +47,6 @@ __constant__ uint16_t c_dist_base[30] = {1,    2,    3,    4,    5,    7,    9, 
                                          33,   49,   65,   97,   129,  193,  257,  385,   513,   769,
                                          1025, 1537, 2049, 3073, 4097, 6145, 8193, 12289, 16385, 24577};
 __constant__ uint8_t c_meta_order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
-
-constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -177,6 +175,20 @@ __device__ void walk_summary(const uint32_t *tokL, uint16_t *Ja, uint16_t *Ca, u
                              uint32_t *out /* SPAN entries */) {
     const uint32_t tid = threadIdx.x;
     const uint32_t absorb = (!strict && !last_seg) ? L - 1 : 0xFFFFu;
+    {
+        // a segment without matches is walked position by position
+        int anym = 0;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const uint32_t x = tid * PER + i;
+            if (x < L) anym |= (tokL[x] >> 16) != 0;
+        }
+        if (!__syncthreads_or(anym)) {
+            for (uint32_t x = tid; x < L && x < SPAN; x += BLOCK) out[x] = L | ((L - x) << 16);
+            __syncthreads();
+            return;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < PER; i++) {
         uint32_t x = tid * PER + i;
@@ -225,7 +237,142 @@ __device__ void walk_summary(const uint32_t *tokL, uint16_t *Ja, uint16_t *Ca, u
 }
 
 // ---------------------------------------------------------------------------
-// k_match_find
+// k_seg_table: per 4 KiB segment, the oldest and newest position of every
+// hash3 value in the segment -- the reference's seg_oldest/seg_newest pair
+// (lz77_hash_build.wgsl:45-54) restricted to the hashes that occur -- as a
+// compact list of (hash | first << 16, last) entries (offsets in the
+// segment), built once per segment and streamed by the 9 segments whose
+// window covers it.  Also computes the segment's raw CRC-32.
+
+constexpr int K0_T = 512;
+constexpr uint32_t TSLOTS = 8192;
+constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+
+// hash3 of lz77_hash_build.wgsl:36-43 on three bytes
+__device__ __forceinline__ uint32_t hash3(uint32_t b0, uint32_t b1, uint32_t b2) {
+    return ((b0 << 16 | b1 << 8 | b2) * 0x9E3779B1u) >> 16;
+}
+
+__global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
+    __shared__ uint32_t keys[TSLOTS];
+    __shared__ uint32_t acc[TSLOTS];
+    __shared__ uint32_t crc_tab[256];
+    __shared__ uint32_t tmp[16];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t G = w.counts->nseg;
+    for (uint32_t i = tid; i < 256; i += K0_T) crc_tab[i] = c_crc_tab[i];
+    for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
+        const MemberDesc md = w.mem[w.seg_mem[g]];
+        const uint32_t n = md.n, s = g - md.seg_base;
+        const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
+        // bytes x0 .. x0+11 of the segment (buffers are padded past every member)
+        const uint32_t x0 = tid * 8;
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(w.in + md.in_off + S + x0);
+        uint32_t d0 = 0, d1 = 0, d2 = 0;
+        if (x0 < L) {
+            d0 = src[0];
+            d1 = src[1];
+            d2 = src[2];
+        }
+        const uint64_t lo = (uint64_t)d0 | ((uint64_t)d1 << 32);
+        for (uint32_t i = tid; i < TSLOTS; i += K0_T) {
+            keys[i] = EMPTY;
+            acc[i] = 0;
+        }
+        __syncthreads();
+        uint32_t slot[8];
+        bool hv[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t x = x0 + i;
+            hv[i] = x < L && S + x + 2 < n;
+            uint32_t b0 = (uint32_t)(lo >> (8 * i)) & 0xFF;
+            uint32_t b1 = i + 1 < 8 ? (uint32_t)(lo >> (8 * (i + 1))) & 0xFF : d2 & 0xFF;
+            uint32_t b2 = i + 2 < 8 ? (uint32_t)(lo >> (8 * (i + 2))) & 0xFF : (d2 >> (8 * (i + 2 - 8))) & 0xFF;
+            uint32_t h = hash3(b0, b1, b2);
+            slot[i] = 0;
+            if (hv[i]) {
+                uint32_t sl = h & (TSLOTS - 1);
+                for (;;) {
+                    uint32_t prev = atomicCAS(&keys[sl], EMPTY, h);
+                    if (prev == EMPTY || prev == h) break;
+                    sl = (sl + 1) & (TSLOTS - 1);
+                }
+                slot[i] = sl;
+                atomicMax(&acc[sl], x + 1);
+            }
+        }
+        __syncthreads();
+        uint32_t last[TSLOTS / K0_T];
+#pragma unroll
+        for (int r = 0; r < (int)(TSLOTS / K0_T); r++) last[r] = acc[tid + r * K0_T];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < (int)(TSLOTS / K0_T); r++) acc[tid + r * K0_T] = EMPTY;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            if (hv[i]) atomicMin(&acc[slot[i]], x0 + i);
+        __syncthreads();
+        {
+            // compact the distinct hashes of the segment: (hash | first << 16, last)
+            uint32_t cnt = 0;
+#pragma unroll
+            for (int r = 0; r < (int)(TSLOTS / K0_T); r++) cnt += keys[tid + r * K0_T] != EMPTY;
+            uint32_t total;
+            uint32_t idx = block_scan<K0_T>(cnt, tmp, &total);
+            uint2 *tab = reinterpret_cast<uint2 *>(w.segtab) + (uint64_t)g * SEG;
+#pragma unroll
+            for (int r = 0; r < (int)(TSLOTS / K0_T); r++) {
+                const uint32_t sl = tid + r * K0_T;
+                const uint32_t k = keys[sl];
+                if (k != EMPTY) tab[idx++] = make_uint2(k | (acc[sl] << 16), last[r] - 1);
+            }
+            if (tid == 0) w.seg_ntab[g] = total;
+        }
+        // raw CRC-32 of the segment: 8 bytes per thread, folded by x^(8 * bytes after them)
+        {
+            uint32_t c = 0;
+            const uint64_t hi = d2;
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                if (x0 + b < L) {
+                    uint32_t byte = (uint32_t)((b < 8 ? lo : hi) >> (8 * b)) & 0xFF;
+                    c = crc_tab[(c ^ byte) & 0xFF] ^ (c >> 8);
+                }
+            }
+            if (x0 < L) {
+                const uint32_t after = L - min(x0 + 8, L);
+                c = gf2_mul(c, L == SEG ? c_crc_fold[tid] : xpow8(after));
+            } else {
+                c = 0;
+            }
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) c ^= __shfl_xor(c, d, 64);
+            if ((tid & 63) == 0) tmp[tid >> 6] = c;
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t t = 0;
+                for (int i = 0; i < K0_T / 64; i++) t ^= tmp[i];
+                w.seg_crc[g] = t;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_match_find: per-position tokens for one 4 KiB segment.
+//
+// The segment's window (32 KiB back, the segment, 258+ bytes ahead) is staged
+// in LDS.  The distinct hashes of the segment (its compact table) get dense ids
+// by rank in a 65536-bit presence map.  Then for k = 0..8 (the reference's
+// segment order, lz77_hash_lookup.wgsl:85-112; older segments only give
+// distances > 32768, rejected at :50) the compact table of segment s-k is
+// streamed in -- prefetched into registers one step ahead -- and every entry
+// whose hash occurs in the segment stores its (first, last) under the id,
+// tagged with k; each position then tries newest, then oldest while
+// best < 16, stopping at 258 (:96-110).  65 KB of LDS: two workgroups per CU.
 
 constexpr int K1_T = 1024;
 constexpr int WIN_WORDS = 9344;  // 37,376 bytes: 32 KiB back + 4 KiB segment + 258 + slack
@@ -235,52 +382,84 @@ struct MatchState {
     uint32_t len, dist;
 };
 
-__device__ __forceinline__ void try_cand(const uint32_t *win, uint32_t WB, uint32_t n, uint32_t p,
-                                         uint32_t cand, uint32_t window, MatchState &st) {
-    if (cand >= p) return;
-    uint32_t dist = p - cand;
-    if (dist > window) return;
-    uint32_t xp = p - WB, xc = cand - WB;
-    uint32_t a = ld4(win, xp), b = ld4(win, xc);
-    if ((a ^ b) & 0x00FFFFFFu) return;
+// try_candidate (lz77_hash_lookup.wgsl:45-67) on the LDS window.  `d0` is the
+// xor of the first four bytes at p and cand; `known` (>= 3 or 0) is a length
+// over which (p, cand) is already known to agree.  st = best len << 16 | dist.
+__device__ __forceinline__ uint32_t try_cand(const uint32_t *win, uint32_t WB, uint32_t n, uint32_t p,
+                                             uint32_t cand, uint32_t window, uint32_t d0, uint32_t known,
+                                             uint32_t st) {
+    if (cand >= p) return st;
+    const uint32_t dist = p - cand;
+    if (dist > window) return st;
+    const uint32_t xp = p - WB, xc = cand - WB;
     uint32_t lim = n - p;
     if (lim > MAX_MATCH) lim = MAX_MATCH;
-    uint32_t len = 3;
-    uint32_t d = (a ^ b);
-    if (d) {
-        len = __builtin_ctz(d) >> 3;
-    } else {
-        len = 4;
+    uint32_t len = known;
+    bool extend = true;
+    if (!known) {
+        if (d0 & 0x00FFFFFFu) return st;
+        len = d0 ? 3 : 4;
+        extend = d0 == 0;
+    }
+    if (extend) {
+        // 8 bytes per step from aligned 64-bit LDS words
+        const uint64_t *w64 = reinterpret_cast<const uint64_t *>(win);
+        const uint32_t ap = xp + len, ac = xc + len;
+        uint32_t qa = ap >> 3, qc = ac >> 3;
+        const uint32_t sa = (ap & 7u) * 8, sc = (ac & 7u) * 8;
+        uint64_t a0 = w64[qa], c0 = w64[qc];
         while (len < lim) {
-            uint32_t e = ld4(win, xp + len) ^ ld4(win, xc + len);
-            if (e) {
-                len += __builtin_ctz(e) >> 3;
+            const uint64_t a1 = w64[++qa], c1 = w64[++qc];
+            const uint64_t av = sa ? (a0 >> sa) | (a1 << (64 - sa)) : a0;
+            const uint64_t cv = sc ? (c0 >> sc) | (c1 << (64 - sc)) : c0;
+            const uint64_t x = av ^ cv;
+            if (x) {
+                len += (uint32_t)__builtin_ctzll(x) >> 3;
                 break;
             }
-            len += 4;
+            len += 8;
+            a0 = a1;
+            c0 = c1;
         }
     }
     if (len > lim) len = lim;
-    if (len > st.len) {
-        st.len = len;
-        st.dist = dist;
+    return len > (st >> 16) ? (len << 16) | dist : st;
+}
+
+// Entries of a segment's compact table handled by one thread (D <= 4096).
+struct Ent4 {
+    uint2 e[4];
+    uint32_t D;
+};
+
+__device__ __forceinline__ void load_ent(const uint2 *tabs, const uint32_t *ntab, uint32_t gj, uint32_t tid, Ent4 &o) {
+    o.D = ntab[gj];
+    const uint2 *T = tabs + (uint64_t)gj * SEG;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const uint32_t i = tid + r * K1_T;
+        o.e[r] = i < o.D ? T[i] : make_uint2(0, 0);
     }
 }
 
-__global__ __launch_bounds__(K1_T) void k_match_find(Work w, int do_summary) {
-    __shared__ uint32_t win[WIN_WORDS];
-    __shared__ uint32_t keys[KSLOTS];
-    __shared__ uint32_t fl[2 * 4096];
-    __shared__ uint32_t crc_tab[256];
-    __shared__ uint32_t tmp[32];
+// The distinct hashes of the query segment get dense ids by rank: a 65536-bit
+// presence map plus the number of set bits before each 32-bit word.
+__device__ __forceinline__ bool has_hash(const uint32_t *bits, uint32_t h) { return (bits[h >> 5] >> (h & 31)) & 1u; }
+__device__ __forceinline__ uint32_t hash_rank(const uint32_t *bits, const uint16_t *rnk, uint32_t h) {
+    return rnk[h >> 5] + __popc(bits[h >> 5] & ((1u << (h & 31)) - 1u));
+}
 
+__global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) {
+    __shared__ __align__(16) uint32_t win[WIN_WORDS];
+    __shared__ uint32_t bits[2048];  // which hashes occur in the segment
+    __shared__ uint16_t rnk[2048];   // set bits before each word of `bits`
+    __shared__ uint32_t fl[SEG];     // per hash id: tag << 24 | first << 12 | last
+    __shared__ uint32_t tmp[32];
     const uint32_t tid = threadIdx.x;
     const uint32_t G = w.counts->nseg;
-    for (uint32_t i = tid; i < 256; i += K1_T) crc_tab[i] = c_crc_tab[i];
-
+    const uint2 *tabs = reinterpret_cast<const uint2 *>(w.segtab);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
-        const uint32_t m = w.seg_mem[g];
-        const MemberDesc md = w.mem[m];
+        const MemberDesc md = w.mem[w.seg_mem[g]];
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2;
         const uint32_t E = min(S + SEG, n), L = E - S;
@@ -288,19 +467,22 @@ __global__ __launch_bounds__(K1_T) void k_match_find(Work w, int do_summary) {
         const uint32_t WE = min(n, E + 272u);
         const uint32_t nb = WE - WB;
         const uint8_t *src = w.in + md.in_off + WB;
+        const uint32_t kmax = min(s, 8u);
+        Ent4 own;
+        load_ent(tabs, w.seg_ntab, g, tid, own);
 
         // ---- window -> LDS (zero beyond the member end)
         for (uint32_t v = tid; v < WIN_WORDS / 4; v += K1_T) {
-            uint32_t b0 = v * 16u;
+            const uint32_t b0 = v * 16u;
             uint4 q = make_uint4(0, 0, 0, 0);
             if (b0 < nb) {
                 q = *reinterpret_cast<const uint4 *>(src + b0);
                 if (b0 + 16u > nb) {
-                    uint32_t rem = nb - b0;
+                    const uint32_t rem = nb - b0;
                     uint32_t *qq = reinterpret_cast<uint32_t *>(&q);
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
-                        uint32_t lo = 4u * j;
+                        const uint32_t lo = 4u * j;
                         if (lo >= rem) qq[j] = 0;
                         else if (lo + 4 > rem) qq[j] &= (1u << (8 * (rem - lo))) - 1u;
                     }
@@ -308,120 +490,116 @@ __global__ __launch_bounds__(K1_T) void k_match_find(Work w, int do_summary) {
             }
             reinterpret_cast<uint4 *>(win)[v] = q;
         }
-        for (uint32_t i = tid; i < KSLOTS; i += K1_T) keys[i] = EMPTY;
+        for (uint32_t i = tid; i < 2048; i += K1_T) bits[i] = 0;
         __syncthreads();
 
-        // ---- own positions: hashes, key table insert
+        // ---- presence map of the segment's own hashes, then ranks
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            if (tid + r * K1_T < own.D) {
+                const uint32_t h = own.e[r].x & 0xFFFFu;
+                atomicOr(&bits[h >> 5], 1u << (h & 31));
+            }
+        __syncthreads();
+        {
+            const uint32_t b0 = bits[2 * tid], b1 = bits[2 * tid + 1];
+            uint32_t tot;
+            const uint32_t ex = block_scan<K1_T>(__popc(b0) + __popc(b1), tmp, &tot);
+            rnk[2 * tid] = (uint16_t)ex;
+            rnk[2 * tid + 1] = (uint16_t)(ex + __popc(b0));
+        }
+        __syncthreads();
+        // k = 0 candidates straight from the own table
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            if (tid + r * K1_T < own.D) {
+                const uint2 e = own.e[r];
+                fl[hash_rank(bits, rnk, e.x & 0xFFFFu)] = ((e.x >> 16) << 12) | e.y;  // tag 0
+            }
+
+        // ---- own positions: hash -> id
         const uint32_t x0 = tid * 4u;
         const uint32_t lx = S - WB + x0;
-        uint64_t vv = (uint64_t)win[lx >> 2] | ((uint64_t)win[(lx >> 2) + 1] << 32);
-        uint32_t hq[4], slot[4], qid[4];
-        bool hv[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            uint32_t x = x0 + i;
-            hv[i] = x < L && S + x + 2u < n;
-            uint32_t key = (uint32_t)((vv >> (8 * i)) & 0xFF) << 16 | (uint32_t)((vv >> (8 * i + 8)) & 0xFF) << 8 |
-                           (uint32_t)((vv >> (8 * i + 16)) & 0xFF);
-            hq[i] = (key * 0x9E3779B1u) >> 16;
-            slot[i] = 0;
-            if (hv[i]) {
-                uint32_t sl = hq[i] & (KSLOTS - 1);
-                for (;;) {
-                    uint32_t prev = atomicCAS(&keys[sl], EMPTY, hq[i]);
-                    if (prev == EMPTY || prev == hq[i]) break;
-                    sl = (sl + 1) & (KSLOTS - 1);
-                }
-                slot[i] = sl;
-            }
-        }
-        __syncthreads();
-        // dense ids for the distinct query hashes
+        uint32_t qid[4];
+        uint32_t hvm = 0;  // bit i: position x0+i can start a match (p + 2 < n)
         {
-            uint32_t cnt = 0;
-#pragma unroll
-            for (int i = 0; i < 8; i++) cnt += keys[tid * 8 + i] != EMPTY;
-            uint32_t D;
-            uint32_t id = block_scan<K1_T>(cnt, tmp, &D);
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                uint32_t kv = keys[tid * 8 + i];
-                if (kv != EMPTY) keys[tid * 8 + i] = (kv & 0xFFFFu) | (id++ << 16);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < 4; i++) qid[i] = hv[i] ? keys[slot[i]] >> 16 : 0;
-            tmp[31] = D;
-        }
-        __syncthreads();
-        const uint32_t D = tmp[31];
-
-        MatchState st[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) st[i] = {0, 0};
-
-        // ---- back segments, newest segment first (lz77_hash_lookup.wgsl:85-112)
-        for (uint32_t k = 0; k <= 8 && k <= s; k++) {
-            const uint32_t j = s - k;
-            for (uint32_t i = tid; i < D; i += K1_T) {
-                fl[i] = EMPTY;
-                fl[4096 + i] = 0;
-            }
-            __syncthreads();
-            if (k == 0) {
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    if (hv[i]) {
-                        atomicMin(&fl[qid[i]], S + x0 + i);
-                        atomicMax(&fl[4096 + qid[i]], S + x0 + i);
-                    }
-            } else {
-                const uint32_t q0 = (j << SEG_LOG2) + x0;  // segment j is full (j < s)
-                const uint32_t bx = q0 - WB;
-                uint64_t bv = (uint64_t)win[bx >> 2] | ((uint64_t)win[(bx >> 2) + 1] << 32);
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    if (q0 + i + 2u >= n) continue;
-                    uint32_t key = (uint32_t)((bv >> (8 * i)) & 0xFF) << 16 |
-                                   (uint32_t)((bv >> (8 * i + 8)) & 0xFF) << 8 | (uint32_t)((bv >> (8 * i + 16)) & 0xFF);
-                    uint32_t h = (key * 0x9E3779B1u) >> 16;
-                    uint32_t sl = h & (KSLOTS - 1);
-                    for (;;) {
-                        uint32_t kv = keys[sl];
-                        if (kv == EMPTY) break;
-                        if ((kv & 0xFFFFu) == h) {
-                            uint32_t id = kv >> 16;
-                            atomicMin(&fl[id], q0 + i);
-                            atomicMax(&fl[4096 + id], q0 + i);
-                            break;
-                        }
-                        sl = (sl + 1) & (KSLOTS - 1);
-                    }
-                }
-            }
-            __syncthreads();
-            int active = 0;
+            const uint64_t vv = (uint64_t)win[lx >> 2] | ((uint64_t)win[(lx >> 2) + 1] << 32);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                if (!hv[i] || st[i].len >= MAX_MATCH) continue;
-                uint32_t f = fl[qid[i]];
-                if (f != EMPTY) {
-                    uint32_t l = fl[4096 + qid[i]];
-                    uint32_t p = S + x0 + i;
-                    try_cand(win, WB, n, p, l, w.window, st[i]);
-                    if (st[i].len < MAX_MATCH && st[i].len < OLDEST_SKIP) try_cand(win, WB, n, p, f, w.window, st[i]);
-                }
-                active |= st[i].len < MAX_MATCH;
+                const uint32_t x = x0 + i;
+                const bool hv = x < L && S + x + 2u < n;
+                const uint32_t h = hash3((uint32_t)(vv >> (8 * i)) & 0xFF, (uint32_t)(vv >> (8 * i + 8)) & 0xFF,
+                                         (uint32_t)(vv >> (8 * i + 16)) & 0xFF);
+                qid[i] = hv ? hash_rank(bits, rnk, h) : 0;
+                hvm |= (uint32_t)hv << i;
             }
+        }
+        __syncthreads();
+
+        uint32_t st[4] = {0, 0, 0, 0};  // best len << 16 | dist per position
+        for (uint32_t k = 0; k <= kmax; k++) {
+            if (k) {
+                // stream segment s-k's entries
+                Ent4 now;
+                load_ent(tabs, w.seg_ntab, g - k, tid, now);
+                const uint32_t tag = k << 24;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    if (tid + r * K1_T >= now.D) continue;
+                    const uint2 e = now.e[r];
+                    const uint32_t h = e.x & 0xFFFFu;
+                    if (has_hash(bits, h)) fl[hash_rank(bits, rnk, h)] = tag | ((e.x >> 16) << 12) | e.y;
+                }
+                __syncthreads();
+            }
+            // evaluate segment s-k: gather both candidates' first bytes for all
+            // four positions, then apply the reference order position by
+            // position (arrays rotate so the loop body is not unrolled)
+            const uint32_t base = (s - k) << SEG_LOG2;
+            uint32_t fv[4], wn[4], wo[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t v = (((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH) ? fl[qid[i]] : EMPTY;
+                fv[i] = v;
+                const bool ok = (v >> 24) == k;
+                const uint32_t p = S + x0 + i;
+                const uint32_t cn = base + (v & 0xFFFu), co = base + ((v >> 12) & 0xFFFu);
+                wn[i] = (ok && cn < p && p - cn <= w.window) ? ld4(win, cn - WB) : 0;
+                wo[i] = (ok && co < p && p - co <= w.window) ? ld4(win, co - WB) : 0;
+            }
+            uint32_t prev = 0;  // state of the previous position (prefix reuse)
+#pragma unroll 1
+            for (uint32_t i = 0; i < 4; i++) {
+                const uint32_t v = fv[0];
+                uint32_t cs = st[0];
+                if ((v >> 24) == k) {
+                    const uint32_t p = S + x0 + i;
+                    const uint32_t own4 = ld4(win, lx + i);
+                    const uint32_t cn = base + (v & 0xFFFu), co = base + ((v >> 12) & 0xFFFu);
+                    const uint32_t hd = prev & 0xFFFFu;
+                    const uint32_t hk = (prev >> 16) >= 4 ? (prev >> 16) - 1 : 0;
+                    cs = try_cand(win, WB, n, p, cn, w.window, own4 ^ wn[0], (p - cn == hd) ? hk : 0, cs);
+                    if ((cs >> 16) < OLDEST_SKIP)
+                        cs = try_cand(win, WB, n, p, co, w.window, own4 ^ wo[0], (p - co == hd) ? hk : 0, cs);
+                }
+                prev = cs;
+                fv[0] = fv[1]; fv[1] = fv[2]; fv[2] = fv[3];
+                wn[0] = wn[1]; wn[1] = wn[2]; wn[2] = wn[3];
+                wo[0] = wo[1]; wo[1] = wo[2]; wo[2] = wo[3];
+                st[0] = st[1]; st[1] = st[2]; st[2] = st[3]; st[3] = cs;
+            }
+            int active = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) active |= ((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH;
             if (!__syncthreads_or(active)) break;
         }
 
         // ---- tokens
         uint32_t t4[4];
+        {
+            const uint32_t ob = ld4(win, lx);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            uint32_t byte = (uint32_t)((vv >> (8 * i)) & 0xFF);
-            t4[i] = (hv[i] && st[i].len) ? ((st[i].len << 16) | st[i].dist) : byte;
+            for (int i = 0; i < 4; i++) t4[i] = st[i] ? st[i] : (ob >> (8 * i)) & 0xFF;
         }
         uint32_t *tg = w.tok + md.pos_base + S + x0;
         if (x0 + 4 <= L) {
@@ -431,42 +609,7 @@ __global__ __launch_bounds__(K1_T) void k_match_find(Work w, int do_summary) {
             for (int i = 0; i < 4; i++)
                 if (x0 + i < L) tg[i] = t4[i];
         }
-
-        // ---- raw CRC of the segment: 256 threads x 16 bytes, right-aligned to
-        // a virtual 4096-byte block (leading zero bytes leave a zero register)
-        {
-            uint32_t c = 0;
-            if (tid < 256) {
-                int pad = (int)SEG - (int)L;
-                for (int b = 0; b < 16; b++) {
-                    int r = (int)(tid * 16 + b) - pad;
-                    if (r < 0) continue;
-                    uint32_t x = S - WB + (uint32_t)r;
-                    uint32_t byte = (win[x >> 2] >> (8 * (x & 3))) & 0xFF;
-                    c = crc_tab[(c ^ byte) & 0xFF] ^ (c >> 8);
-                }
-                c = gf2_mul(c, c_crc_fold[tid]);
-            }
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) c ^= __shfl_xor(c, d, 64);
-            if ((tid & 63) == 0 && tid < 256) tmp[tid >> 6] = c;
-            __syncthreads();
-            if (tid == 0) w.seg_crc[g] = tmp[0] ^ tmp[1] ^ tmp[2] ^ tmp[3];
-        }
-
-        // ---- walk summary
-        if (do_summary) {
-            uint32_t *tokL = keys;
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (x0 + i < L) tokL[x0 + i] = t4[i];
-            __syncthreads();
-            uint16_t *J = reinterpret_cast<uint16_t *>(fl);
-            walk_summary<K1_T, 4>(tokL, J, J + 4096, J + 8192, J + 12288, L, E == n, w.strict,
-                                  w.wsum + (uint64_t)g * SPAN);
-        } else {
-            __syncthreads();
-        }
+        (void)do_summary;
     }
 }
 
@@ -696,11 +839,11 @@ __global__ __launch_bounds__(1024) void k_hist_tokens(Work w) {
 
 struct HuffScratch {
     uint32_t work[288];
-    uint32_t ifreq[288];
+    uint32_t sf[288];       // leaf frequencies in (freq, id) order
+    uint32_t ifreq[288];    // internal node frequencies, creation order
     uint16_t sorted[288];
-    uint16_t lparent[288];
-    uint16_t iparent[288];
-    uint16_t idepth[288];
+    uint16_t par[2][576];   // node parent; leaf i -> node i, internal k -> node 288 + k
+    uint16_t dep[2][576];
     uint32_t misc[4];
 };
 
@@ -708,11 +851,11 @@ struct HuffScratch {
 // (freq, id) from a binary heap whose internal nodes get ids n, n+1, ...;
 // merged frequencies never decrease and internal ids exceed every leaf id,
 // so merging a (freq, id)-sorted leaf list with a FIFO of internal nodes
-// (leaf first on equal frequency) pops the same sequence.
+// (leaf first on equal frequency) pops the same sequence.  The merge runs on
+// lane 0 from registers; depths (= code lengths) come from pointer jumping
+// over the parent links on all 64 lanes.
 __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens, HuffScratch &hs) {
     const int lane = threadIdx.x;
-    if (lane == 0) hs.misc[0] = 0;
-    __syncthreads();
     uint32_t ucnt = 0;
     for (int i = lane; i < n; i += 64) {
         hs.work[i] = f[i];
@@ -726,49 +869,73 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
         __syncthreads();
         return;
     }
+    const uint32_t root = 288 + used - 2;
     for (;;) {
         for (int i = lane; i < n; i += 64) {
-            uint32_t wi = hs.work[i];
+            const uint32_t wi = hs.work[i];
             if (!wi) continue;
             uint32_t r = 0;
             for (int j = 0; j < n; j++) {
-                uint32_t wj = hs.work[j];
+                const uint32_t wj = hs.work[j];
                 r += (wj != 0) & ((wj < wi) | ((wj == wi) & (j < i)));
             }
             hs.sorted[r] = (uint16_t)i;
+            hs.sf[r] = wi;
         }
         __syncthreads();
         if (lane == 0) {
             uint32_t nl = 0, qh = 0;
+            uint32_t lf = hs.sf[0];
+            uint32_t lid = hs.sorted[0];
             for (uint32_t k = 0; k + 1 < used; k++) {
                 uint32_t fsum = 0;
+#pragma unroll
                 for (int pick = 0; pick < 2; pick++) {
-                    bool leaf = nl < used && (qh >= k || hs.work[hs.sorted[nl]] <= hs.ifreq[qh]);
-                    if (leaf) {
-                        uint32_t id = hs.sorted[nl++];
-                        fsum += hs.work[id];
-                        hs.lparent[id] = (uint16_t)k;
+                    const bool have_int = qh < k;
+                    const uint32_t qf = have_int ? hs.ifreq[qh] : 0;
+                    if (nl < used && (!have_int || lf <= qf)) {
+                        fsum += lf;
+                        hs.par[0][lid] = (uint16_t)(288 + k);
+                        nl++;
+                        if (nl < used) {
+                            lf = hs.sf[nl];
+                            lid = hs.sorted[nl];
+                        }
                     } else {
-                        uint32_t id = qh++;
-                        fsum += hs.ifreq[id];
-                        hs.iparent[id] = (uint16_t)k;
+                        fsum += qf;
+                        hs.par[0][288 + qh] = (uint16_t)(288 + k);
+                        qh++;
                     }
                 }
                 hs.ifreq[k] = fsum;
             }
-            hs.idepth[used - 2] = 0;
-            for (int k = (int)used - 3; k >= 0; k--) hs.idepth[k] = hs.idepth[hs.iparent[k]] + 1;
+            hs.par[0][root] = (uint16_t)root;
         }
         __syncthreads();
-        uint32_t mx = 0;
-        for (int i = lane; i < n; i += 64) {
-            uint32_t d = hs.work[i] ? (uint32_t)hs.idepth[hs.lparent[i]] + 1 : 0;
-            mx = max(mx, d);
+        // depth of every node: pointer jumping, ceil(log2(286)) = 9 rounds
+        for (int id = lane; id < 576; id += 64) {
+            const bool valid = id < 288 ? (id < n && hs.work[id] != 0) : (uint32_t)(id - 288) + 1 < used;
+            if (valid) hs.dep[0][id] = id == (int)root ? 0 : 1;
         }
+        __syncthreads();
+        int cur = 0;
+        for (int r = 0; r < 9; r++) {
+            for (int id = lane; id < 576; id += 64) {
+                const bool valid = id < 288 ? (id < n && hs.work[id] != 0) : (uint32_t)(id - 288) + 1 < used;
+                if (!valid) continue;
+                const uint32_t pa = hs.par[cur][id];
+                hs.dep[cur ^ 1][id] = hs.dep[cur][id] + hs.dep[cur][pa];
+                hs.par[cur ^ 1][id] = hs.par[cur][pa];
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+        uint32_t mx = 0;
+        for (int i = lane; i < n; i += 64)
+            if (hs.work[i]) mx = max(mx, (uint32_t)hs.dep[cur][i]);
         for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
         if (mx <= (uint32_t)limit) {
-            for (int i = lane; i < n; i += 64)
-                lens[i] = hs.work[i] ? (uint8_t)(hs.idepth[hs.lparent[i]] + 1) : 0;
+            for (int i = lane; i < n; i += 64) lens[i] = hs.work[i] ? (uint8_t)hs.dep[cur][i] : 0;
             __syncthreads();
             return;
         }
@@ -779,38 +946,60 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
     }
 }
 
-// canonical codes (huffman.rs:122-152), stored bit-reversed (deflate.rs:876-884)
+// canonical codes (huffman.rs:122-152), stored bit-reversed with the length
+// (deflate.rs:876-884).  All 64 lanes: per-length counts and in-length ranks
+// from ballots.
 __device__ void canonical_rev(const uint8_t *lens, int n, uint32_t *codes) {
-    uint32_t bl_count[16], next_code[16];
-    for (int b = 0; b < 16; b++) bl_count[b] = 0;
-    for (int i = 0; i < n; i++) bl_count[lens[i]]++;
-    bl_count[0] = 0;
+    const int lane = threadIdx.x;
+    const uint64_t lt = (1ull << lane) - 1;
+    uint32_t cnt[16];
+#pragma unroll
+    for (int b = 0; b < 16; b++) cnt[b] = 0;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const uint32_t l = c0 + lane < n ? lens[c0 + lane] : 0;
+#pragma unroll
+        for (int b = 1; b < 16; b++) cnt[b] += __popcll(__ballot(l == (uint32_t)b));
+    }
+    uint32_t next[16];
     uint32_t code = 0;
-    next_code[0] = 0;
+    next[0] = 0;
+#pragma unroll
     for (int b = 1; b < 16; b++) {
-        code = (code + bl_count[b - 1]) << 1;
-        next_code[b] = code;
+        code = (code + (b > 1 ? cnt[b - 1] : 0)) << 1;
+        next[b] = code;
     }
-    for (int i = 0; i < n; i++) {
-        uint32_t l = lens[i];
-        if (!l) {
-            codes[i] = 0;
-            continue;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const uint32_t l = c0 + lane < n ? lens[c0 + lane] : 0;
+        uint32_t mine = 0;
+#pragma unroll
+        for (int b = 1; b < 16; b++) {
+            const uint64_t m = __ballot(l == (uint32_t)b);
+            if (l == (uint32_t)b) mine = next[b] + __popcll(m & lt);
+            next[b] += __popcll(m);
         }
-        uint32_t c = next_code[l]++;
-        codes[i] = (__builtin_bitreverse32(c) >> (32 - l)) | (l << 16);
+        if (c0 + lane < n) codes[c0 + lane] = l ? ((__builtin_bitreverse32(mine) >> (32 - l)) | (l << 16)) : 0;
     }
+    __syncthreads();
 }
 
+// LSB-first bit writer with a register accumulator (lane 0 only).
 struct BitBuf {
     uint32_t *words;
-    uint32_t nbits;
+    uint64_t acc = 0;
+    uint32_t nacc = 0, nw = 0;
     __device__ void put(uint32_t v, uint32_t nb) {
-        if (!nb) return;
-        uint32_t wi = nbits >> 5, sh = nbits & 31;
-        words[wi] |= v << sh;
-        if (sh + nb > 32) words[wi + 1] |= v >> (32 - sh);
-        nbits += nb;
+        acc |= (uint64_t)v << nacc;
+        nacc += nb;
+        if (nacc >= 32) {
+            words[nw++] = (uint32_t)acc;
+            acc >>= 32;
+            nacc -= 32;
+        }
+    }
+    __device__ uint32_t finish() {
+        const uint32_t bits = nw * 32 + nacc;
+        if (nacc) words[nw++] = (uint32_t)acc;
+        return bits;
     }
 };
 
@@ -831,36 +1020,39 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
     for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
         const MemberDesc md = w.mem[m];
         for (int i = lane; i < (int)NSYM; i += 64) freq[i] = w.hist[(uint64_t)m * NSYM + i];
-        for (int i = lane; i < (int)PREFIX_WORDS + 2; i += 64) pbuf[i] = 0;
         for (int i = lane; i < 20; i += 64) mfreq[i] = 0;
         __syncthreads();
         if (lane == 0) freq[256] += 1;  // end of block
         __syncthreads();
         build_lengths(freq, 286, 15, lens, hs);
         build_lengths(freq + DSYM0, 30, 15, lens + DSYM0, hs);
-        if (lane == 0) {
+        {
             // single/zero distance-code fix-up and litlen fix-up (deflate.rs:855-867)
-            uint32_t used_d = 0, only = 0;
-            for (int i = 0; i < 30; i++)
-                if (lens[DSYM0 + i]) {
-                    if (!used_d) only = i;
-                    used_d++;
-                }
-            if (used_d == 0) {
-                lens[DSYM0] = 1;
-                lens[DSYM0 + 1] = 1;
-            } else if (used_d == 1) {
-                lens[DSYM0 + only] = 1;
-                lens[DSYM0 + 1 - min(only, 1u)] = 1;
-            }
+            const uint64_t dm = __ballot(lane < 30 && lens[DSYM0 + lane] != 0);
             uint32_t used_ll = 0;
-            for (int i = 0; i < 286; i++) used_ll += lens[i] > 0;
-            if (used_ll < 2) {
-                lens[0] = 1;
-                lens[256] = 1;
+            for (int i = lane; i < 286; i += 64) used_ll += lens[i] != 0;
+            for (int d = 32; d >= 1; d >>= 1) used_ll += __shfl_xor(used_ll, d, 64);
+            __syncthreads();
+            if (lane == 0) {
+                const uint32_t used_d = __popcll(dm);
+                if (used_d == 0) {
+                    lens[DSYM0] = 1;
+                    lens[DSYM0 + 1] = 1;
+                } else if (used_d == 1) {
+                    const uint32_t only = __builtin_ctzll(dm);
+                    lens[DSYM0 + only] = 1;
+                    lens[DSYM0 + 1 - min(only, 1u)] = 1;
+                }
+                if (used_ll < 2) {
+                    lens[0] = 1;
+                    lens[256] = 1;
+                }
             }
-            canonical_rev(lens, 286, codes);
-            canonical_rev(lens + DSYM0, 30, codes + DSYM0);
+            __syncthreads();
+        }
+        canonical_rev(lens, 286, codes);
+        canonical_rev(lens + DSYM0, 30, codes + DSYM0);
+        if (lane == 0) {
             // trim_to_min (deflate.rs:329-335) and RLE (deflate.rs:254-326)
             uint32_t nlit = 286, ndist = 30;
             while (nlit > 257 && lens[nlit - 1] == 0) nlit--;
@@ -914,11 +1106,12 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
         }
         __syncthreads();
         build_lengths(mfreq, 19, 7, mlens, hs);
+        canonical_rev(mlens, 19, mcodes);
         if (lane == 0) {
-            canonical_rev(mlens, 19, mcodes);
             uint32_t hclen = 19;
             while (hclen > 4 && mlens[c_meta_order[hclen - 1]] == 0) hclen--;
-            BitBuf bb{pbuf, 0};
+            BitBuf bb;
+            bb.words = pbuf;
             if (w.gzip) {
                 // gzip header 1f 8b 08 00 00000000 00 ff (deflate.rs:965-972)
                 bb.put(0x00088b1fu, 32);
@@ -935,7 +1128,7 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
                 bb.put(mcodes[c] & 0xFFFF, mlens[c]);
                 bb.put(ev, eb);
             }
-            info[3] = bb.nbits;
+            info[3] = bb.finish();
         }
         __syncthreads();
         // publish codes / prefix, count body bits
@@ -1152,7 +1345,7 @@ __global__ __launch_bounds__(K6_T) void k_emit(Work w) {
 namespace gpz {
 
 hipError_t upload_constants() {
-    uint32_t tab[256], fold[256], xp[32];
+    uint32_t tab[256], fold[512], xp[32];
     for (uint32_t i = 0; i < 256; i++) {
         uint32_t c = i;
         for (int k = 0; k < 8; k++) c = (c & 1u) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
@@ -1174,13 +1367,16 @@ hipError_t upload_constants() {
             if (n & 1) r = mul(r, xp[k]);
         return r;
     };
-    for (uint32_t t = 0; t < 256; t++) fold[t] = pw(16ull * (255 - t));
+    for (uint32_t t = 0; t < 512; t++) fold[t] = pw(8ull * (511 - t));
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_tab), tab, sizeof(tab));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_fold), fold, sizeof(fold));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_xpow), xp, sizeof(xp));
     return e;
 }
 
+void launch_seg_table(const Work &w, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_seg_table, dim3(grid), dim3(K0_T), 0, st, w);
+}
 void launch_match_find(const Work &w, uint32_t grid, int summary, hipStream_t st) {
     hipLaunchKernelGGL(k_match_find, dim3(grid), dim3(K1_T), 0, st, w, summary);
 }

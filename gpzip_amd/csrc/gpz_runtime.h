@@ -40,12 +40,12 @@ struct Caps {
 
 struct Layout {
     uint64_t mem, seg_mem, chunk_mem0, counts, desc_end;
-    uint64_t tok, wsum, seg_crc, seg_entry, seg_tokbase, walked, hist, ntok, codes, prefix, prefix_bits;
+    uint64_t tok, segtab, seg_ntab, wsum, seg_crc, seg_entry, seg_tokbase, walked, hist, ntok, codes, prefix, prefix_bits;
     uint64_t mem_bytes, mem_crc, mem_off, chunk_off, chunk_len, total, out, end;
 };
 
 struct Grids {
-    uint32_t k1 = 1, k3 = 1, chain = 1, huff = 1, emit = 1;
+    uint32_t k0 = 1, k1 = 1, k3 = 1, chain = 1, huff = 1, emit = 1;
 };
 
 Grids grids_for(const Caps &c, int n_cu);

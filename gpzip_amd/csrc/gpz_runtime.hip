@@ -28,6 +28,7 @@
 namespace gpz {
 
 hipError_t upload_constants();
+void launch_seg_table(const Work &w, uint32_t grid, hipStream_t st);
 void launch_match_find(const Work &w, uint32_t grid, int summary, hipStream_t st);
 void launch_walk_summary(const Work &w, uint32_t grid, hipStream_t st);
 void launch_chain(const Work &w, uint32_t grid, hipStream_t st);
@@ -171,6 +172,8 @@ int Workspace::reserve(const Caps &want) {
     L.counts = carve(cur, sizeof(Counts));
     L.desc_end = cur;
     L.tok = carve(cur, c.P * 4);
+    L.segtab = carve(cur, (uint64_t)c.G * SEG * 8);
+    L.seg_ntab = carve(cur, c.G * 4);
     L.wsum = carve(cur, (uint64_t)c.G * SPAN * 4);
     L.seg_crc = carve(cur, c.G * 4);
     L.seg_entry = carve(cur, c.G * 4);
@@ -204,6 +207,8 @@ int Workspace::reserve(const Caps &want) {
     w.chunk_mem0 = (const uint32_t *)(b + L.chunk_mem0);
     w.counts = (const Counts *)(b + L.counts);
     w.tok = (uint32_t *)(b + L.tok);
+    w.segtab = (uint64_t *)(b + L.segtab);
+    w.seg_ntab = (uint32_t *)(b + L.seg_ntab);
     w.wsum = (uint32_t *)(b + L.wsum);
     w.seg_crc = (uint32_t *)(b + L.seg_crc);
     w.seg_entry = (uint32_t *)(b + L.seg_entry);
@@ -239,6 +244,7 @@ int Workspace::upload_plan(const Plan &pl, hipStream_t st) {
 
 Grids grids_for(const Caps &c, int n_cu) {
     Grids g;
+    g.k0 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.G, (uint64_t)n_cu * 8));
     g.k1 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.G, (uint64_t)n_cu * 8));
     g.k3 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.G, (uint64_t)n_cu * 8));
     g.chain = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 16));
@@ -253,6 +259,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
     if (ev) hipEventRecord(ev[0], st);
     switch (mode) {
         case MODE_MATCH:
+            launch_seg_table(w, g.k0, st);
             launch_match_find(w, g.k1, 0, st);
             break;
         case MODE_WALK:
@@ -269,8 +276,11 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
             if (ev) hipEventRecord(ev[4], st);
             break;
         default:  // MODE_FULL
-            launch_match_find(w, g.k1, 1, st);
+            launch_seg_table(w, g.k0, st);
+            if (ev) hipEventRecord(ev[5], st);
+            launch_match_find(w, g.k1, 0, st);
             if (ev) hipEventRecord(ev[1], st);
+            launch_walk_summary(w, g.k3, st);
             launch_chain(w, g.chain, st);
             launch_walk_emit(w, g.k3, st);
             launch_huffman(w, g.huff, st);

@@ -165,8 +165,9 @@ int gpz_plan_run(gpz_plan *p, const void *d_in, void *d_out);
 int gpz_plan_sync(gpz_plan *p);
 /* After sync: per-chunk output offset/length in d_out and the total. */
 int gpz_plan_result(gpz_plan *p, uint64_t *chunk_off, uint64_t *chunk_len, uint64_t *total);
-/* Average device time (ms) of the last run's kernels, from HIP events on
- * the plan's stream: [0] whole run, [1] match-find kernel, [2] emit kernel. */
+/* Device time (ms) of the last run, from HIP events on the plan's stream:
+ * [0] whole run, [1] match-find kernel, [2] emit kernel, [3] segment-table
+ * kernel ([1]-[3] only for runs without GPZ_PLAN_GRAPH). */
 int gpz_plan_timing(gpz_plan *p, float *ms, size_t n);
 void gpz_plan_destroy(gpz_plan *p);
 
