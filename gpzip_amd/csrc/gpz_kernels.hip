@@ -161,79 +161,62 @@ __device__ __forceinline__ uint32_t ld4(const uint32_t *win, uint32_t x) {
 }
 
 // ---------------------------------------------------------------------------
-// walk summary (shared by k_match_find and k_walk_summary)
+// walk helpers
 //
-// For every entry offset x of the segment, the first walk position at or
-// beyond the segment end (relative to the segment start) and the number of
-// tokens emitted before it.  When the segment is not the member's last, the
-// lazy rule at its last position depends on the next segment's first token;
-// that position is made absorbing (J = x, C = 0) and k_chain resolves it.
+// The lazy rule (lz77.rs:264-282) gives every position a successor:
+//   literal                                  -> x + 1
+//   match followed by a strictly longer match -> x + 1 (demoted to a literal)
+//   otherwise                                 -> x + len
+// (strict rule, walk.rs:512-525: x + len for every match).  The walk is the
+// orbit of the member's position 0.  Two levels: inside each 64-position
+// block a wave doubles successors with lane shuffles (no barriers) to get,
+// for every position, the first orbit point past the block and the 64-bit
+// set of positions it visits inside the block; across blocks the workgroup
+// follows those block exits (a segment has 64 blocks: at most 64 hops).
 
-template <int BLOCK, int PER>
-__device__ void walk_summary(const uint32_t *tokL, uint16_t *Ja, uint16_t *Ca, uint16_t *Jb,
-                             uint16_t *Cb, uint32_t L, bool last_seg, uint32_t strict,
-                             uint32_t *out /* SPAN entries */) {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t absorb = (!strict && !last_seg) ? L - 1 : 0xFFFFu;
-    {
-        // a segment without matches is walked position by position
-        int anym = 0;
-#pragma unroll
-        for (int i = 0; i < PER; i++) {
-            const uint32_t x = tid * PER + i;
-            if (x < L) anym |= (tokL[x] >> 16) != 0;
+// Successor of segment position x.  next_known: tokL[L] holds the next
+// segment's first token.  absorb_last: the last position's successor depends
+// on the next segment and is left to k_chain (self-loop).
+__device__ __forceinline__ uint32_t succ_of(const uint32_t *tokL, uint32_t x, uint32_t L, bool next_known,
+                                            uint32_t strict, bool absorb_last, bool &dem) {
+    dem = false;
+    const uint32_t t = tokL[x], len = t >> 16;
+    if (len == 0) return x + 1;
+    if (strict) return x + len;
+    if (x + 1 < L || next_known) {
+        const uint32_t nl = tokL[x + 1] >> 16;
+        if (nl != 0 && nl > len) {
+            dem = true;
+            return x + 1;
         }
-        if (!__syncthreads_or(anym)) {
-            for (uint32_t x = tid; x < L && x < SPAN; x += BLOCK) out[x] = L | ((L - x) << 16);
-            __syncthreads();
-            return;
+        return x + len;
+    }
+    return absorb_last ? x : x + len;
+}
+
+// Wave-level doubling inside the 64-position block starting at bs.  J is the
+// lane's successor (segment-relative).  Returns the first orbit point at or
+// past the block end (or the self-loop position) and the visited set.
+__device__ __forceinline__ uint32_t block_walk(uint32_t J, uint32_t bs, uint32_t L, bool valid, bool selfloop,
+                                               uint64_t &mask) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t lim = min(64u, L - bs);
+    uint32_t j = valid ? J - bs : 64u;
+    uint64_t M = (valid && !selfloop) ? (1ull << lane) : 0ull;
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        const bool inside = j < lim;
+        const int src = inside ? (int)j : (int)lane;
+        const uint32_t jj = (uint32_t)__shfl((int)j, src, 64);
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)M, src, 64);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(M >> 32), src, 64);
+        if (inside) {
+            j = jj;
+            M |= ((uint64_t)hi << 32) | lo;
         }
     }
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-        uint32_t x = tid * PER + i;
-        if (x >= L) continue;
-        uint32_t t = tokL[x], len = t >> 16, J, C = 1;
-        if (len == 0) {
-            J = x + 1;
-        } else if (strict) {
-            J = x + len;
-        } else if (x + 1 < L) {
-            uint32_t nl = tokL[x + 1] >> 16;
-            J = (nl != 0 && nl > len) ? x + 1 : x + len;
-        } else if (last_seg) {
-            J = x + len;
-        } else {
-            J = x;  // absorbing
-            C = 0;
-        }
-        Ja[x] = (uint16_t)(J < 0xFFFFu ? J : 0xFFFFu);
-        Ca[x] = (uint16_t)C;
-    }
-    __syncthreads();
-    for (int r = 0; r < 13; r++) {
-        int alive = 0;
-#pragma unroll
-        for (int i = 0; i < PER; i++) {
-            uint32_t x = tid * PER + i;
-            if (x >= L) continue;
-            uint32_t j = Ja[x], c = Ca[x];
-            if (j < L) {
-                c += Ca[j];
-                j = Ja[j];
-            }
-            Jb[x] = (uint16_t)j;
-            Cb[x] = (uint16_t)c;
-            alive |= (j < L && j != absorb);
-        }
-        int any = __syncthreads_or(alive);
-        uint16_t *t;
-        t = Ja; Ja = Jb; Jb = t;
-        t = Ca; Ca = Cb; Cb = t;
-        if (!any) break;
-    }
-    for (uint32_t x = tid; x < L && x < SPAN; x += BLOCK) out[x] = (uint32_t)Ja[x] | ((uint32_t)Ca[x] << 16);
-    __syncthreads();
+    mask = M;
+    return bs + j;
 }
 
 // ---------------------------------------------------------------------------
@@ -613,20 +596,66 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
     }
 }
 
-// Walk summary from tokens already in w.tok (stage-level gpz_walk).
-__global__ __launch_bounds__(K1_T) void k_walk_summary(Work w) {
-    __shared__ uint32_t tokL[4096];
-    __shared__ uint16_t J[4 * 4096];
+// k_walk_summary: for every entry offset x < 258 of a segment, the first walk
+// position at or past the segment end (relative to the segment start) and the
+// number of tokens emitted before it.  When the segment is not the member's
+// last, the lazy rule at its last position depends on the next segment's
+// first token; that position is a self-loop here and k_chain resolves it.
+constexpr int KW_T = 1024;
+
+__global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
+    __shared__ uint32_t tokL[SEG];
+    __shared__ uint16_t Ea[SEG], Ca[SEG];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t G = w.counts->nseg;
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
-        const uint32_t m = w.seg_mem[g];
-        const MemberDesc md = w.mem[m];
+        const MemberDesc md = w.mem[w.seg_mem[g]];
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
-        for (uint32_t x = threadIdx.x; x < L; x += K1_T) tokL[x] = w.tok[md.pos_base + S + x];
+        const bool last_seg = E == n;
+        const bool absorb_last = !w.strict && !last_seg;
+        uint32_t *out = w.wsum + (uint64_t)g * SPAN;
+        int anym = 0;
+        for (uint32_t x = tid; x < L; x += KW_T) {
+            const uint32_t t = w.tok[md.pos_base + S + x];
+            tokL[x] = t;
+            anym |= (t >> 16) != 0;
+        }
+        const int any_match = __syncthreads_or(anym);
+        // the last position is a self-loop only when it holds a match
+        const uint32_t absorb = (absorb_last && (tokL[L - 1] >> 16) != 0) ? L - 1 : 0xFFFFu;
+        if (!any_match) {
+            // no matches: every position is walked
+            for (uint32_t x = tid; x < L && x < SPAN; x += KW_T) out[x] = L | ((L - x) << 16);
+            __syncthreads();
+            continue;
+        }
+        // level 1: 16 waves x 4 blocks
+        for (uint32_t b = wid; b < 64; b += KW_T / 64) {
+            const uint32_t bs = b * 64;
+            if (bs >= L) break;
+            const uint32_t x = bs + lane;
+            const bool valid = x < L;
+            bool dem;
+            const uint32_t J = valid ? succ_of(tokL, x, L, false, w.strict, absorb_last, dem) : 0;
+            uint64_t M;
+            const uint32_t ex = block_walk(J, bs, L, valid, valid && x == absorb, M);
+            if (valid) {
+                Ea[x] = (uint16_t)ex;
+                Ca[x] = (uint16_t)__popcll(M);
+            }
+        }
         __syncthreads();
-        walk_summary<K1_T, 4>(tokL, J, J + 4096, J + 8192, J + 12288, L, E == n, w.strict,
-                              w.wsum + (uint64_t)g * SPAN);
+        // level 2: each entry follows the block exits (at most 64 hops)
+        for (uint32_t x = tid; x < L && x < SPAN; x += KW_T) {
+            uint32_t y = x, c = 0;
+            while (y < L && y != absorb) {
+                c += Ca[y];
+                y = Ea[y];
+            }
+            out[x] = y | (c << 16);
+        }
+        __syncthreads();
     }
 }
 
@@ -695,17 +724,23 @@ __global__ __launch_bounds__(64) void k_chain(Work w) {
 }
 
 // ---------------------------------------------------------------------------
-// k_walk_emit: mark walked positions, compact tokens, histogram
+// k_walk_emit: the walked positions of a segment from its known entry, their
+// tokens compacted into the member's token stream, and the histogram.
+// One lane follows the block exits (level 1) from the entry, marking the
+// orbit's block entries (level 2); each marked entry's visited set comes from
+// a second level-1 pass.
 
 constexpr int K3_T = 1024;
 
 __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
-    __shared__ uint32_t tokL[4096 + 4];
-    __shared__ uint16_t Ja[4096], Jb[4096];
-    __shared__ uint8_t mark[4096];
+    __shared__ uint32_t tokL[SEG + 4];
+    __shared__ uint16_t Ja[SEG];
+    __shared__ uint8_t mark[SEG];
+    __shared__ uint64_t vb[64];
+    __shared__ uint32_t cntb[64];
     __shared__ uint32_t hist[NSYM];
-    __shared__ uint32_t tmp[32];
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint64_t lt = (1ull << lane) - 1;
     const uint32_t G = w.counts->nseg;
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
         const uint32_t m = w.seg_mem[g];
@@ -714,88 +749,87 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
         const uint32_t e = w.seg_entry[g];
         if (e >= L) continue;  // the walk jumps over this segment (uniform)
+        const bool next_known = E < n;
         const uint32_t tokbase = w.seg_tokbase[g];
         const uint32_t *tg = w.tok + md.pos_base + S;
-        const uint32_t x0 = tid * 4;
-        if (x0 + 4 <= L) {
-            uint4 q = *reinterpret_cast<const uint4 *>(tg + x0);
-            tokL[x0] = q.x;
-            tokL[x0 + 1] = q.y;
-            tokL[x0 + 2] = q.z;
-            tokL[x0 + 3] = q.w;
-        } else {
-            for (uint32_t x = x0; x < L; x++) tokL[x] = tg[x];
+        for (uint32_t x = tid; x < L; x += K3_T) {
+            tokL[x] = tg[x];
+            mark[x] = 0;
         }
-        if (tid == 0) tokL[L] = E < n ? tg[L] : 0;
+        if (tid == 0) tokL[L] = next_known ? tg[L] : 0;
         for (uint32_t i = tid; i < NSYM; i += K3_T) hist[i] = 0;
         __syncthreads();
-        uint32_t dem = 0;  // bit i: position x0+i is a match demoted to a literal
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            uint32_t x = x0 + i;
-            if (x >= L) continue;
-            uint32_t t = tokL[x], len = t >> 16, J;
-            if (len == 0) {
-                J = x + 1;
-            } else if (w.strict) {
-                J = x + len;
-            } else if (x + 1 < n - S) {
-                uint32_t nl = tokL[x + 1] >> 16;
-                if (nl != 0 && nl > len) {
-                    J = x + 1;
-                    dem |= 1u << i;
-                } else {
-                    J = x + len;
-                }
-            } else {
-                J = x + len;
-            }
-            Ja[x] = (uint16_t)(J < 0xFFFFu ? J : 0xFFFFu);
-            mark[x] = x == e;
+        // level 1: block exits
+        for (uint32_t b = wid; b < 64; b += K3_T / 64) {
+            const uint32_t bs = b * 64;
+            if (bs >= L) break;
+            const uint32_t x = bs + lane;
+            const bool valid = x < L;
+            bool dem;
+            const uint32_t J = valid ? succ_of(tokL, x, L, next_known, w.strict, false, dem) : 0;
+            uint64_t M;
+            const uint32_t ex = block_walk(J, bs, L, valid, false, M);
+            if (valid) Ja[x] = (uint16_t)ex;
         }
         __syncthreads();
-        uint16_t *A = Ja, *B = Jb;
-        for (int r = 0; r < 13; r++) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                uint32_t x = x0 + i;
-                if (x < L && mark[x]) {
-                    uint32_t j = A[x];
-                    if (j < L) mark[j] = 1;
+        // level 2: one lane follows the block exits from the entry (at most 64 hops)
+        if (tid == 0) {
+            uint32_t y = e;
+            while (y < L) {
+                mark[y] = 1;
+                y = Ja[y];
+            }
+        }
+        __syncthreads();
+        // visited set of each block from its marked entry
+        for (uint32_t b = wid; b < 64; b += K3_T / 64) {
+            const uint32_t bs = b * 64;
+            uint64_t V = 0;
+            if (bs < L) {
+                const uint32_t x = bs + lane;
+                const bool valid = x < L;
+                const uint64_t em = __ballot(valid && mark[x]);
+                if (em) {
+                    bool dem;
+                    const uint32_t J = valid ? succ_of(tokL, x, L, next_known, w.strict, false, dem) : 0;
+                    uint64_t M;
+                    block_walk(J, bs, L, valid, false, M);
+                    const int el = __builtin_ctzll(em);
+                    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)M, el, 64);
+                    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(M >> 32), el, 64);
+                    V = ((uint64_t)hi << 32) | lo;
                 }
             }
-            __syncthreads();
-            int alive = 0;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                uint32_t x = x0 + i;
-                if (x >= L) continue;
-                uint32_t j = A[x];
-                if (j < L) j = A[j];
-                B[x] = (uint16_t)j;
-                alive |= j < L;
+            if (lane == 0) {
+                vb[b] = V;
+                cntb[b] = __popcll(V);
             }
-            int any = __syncthreads_or(alive);
-            uint16_t *t = A;
-            A = B;
-            B = t;
-            if (!any) break;
         }
-        // ranks and output
-        uint32_t cnt = 0;
+        __syncthreads();
+        if (wid == 0) {  // exclusive scan of the 64 block counts
+            const uint32_t c = cntb[lane];
+            uint32_t x = c;
 #pragma unroll
-        for (int i = 0; i < 4; i++) cnt += (x0 + i < L) && mark[x0 + i];
-        uint32_t total;
-        uint32_t rank = block_scan<K3_T>(cnt, tmp, &total);
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, 64);
+                if (lane >= (uint32_t)d) x += y;
+            }
+            cntb[lane] = x - c;
+        }
+        __syncthreads();
         uint32_t *outp = w.walked + md.pos_base + tokbase;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            uint32_t x = x0 + i;
-            if (x >= L || !mark[x]) continue;
+        for (uint32_t b = wid; b < 64; b += K3_T / 64) {
+            const uint32_t bs = b * 64;
+            if (bs >= L) break;
+            const uint64_t V = vb[b];
+            if (!((V >> lane) & 1)) continue;
+            const uint32_t x = bs + lane;
+            bool dem;
+            succ_of(tokL, x, L, next_known, w.strict, false, dem);
             uint32_t t = tokL[x];
-            if (dem >> i & 1) t = w.in[md.in_off + S + x];
-            outp[rank++] = t;
-            uint32_t len = t >> 16, v = t & 0xFFFFu;
+            if (dem) t = w.in[md.in_off + S + x];
+            outp[cntb[b] + __popcll(V & lt)] = t;
+            const uint32_t len = t >> 16, v = t & 0xFFFFu;
             if (len == 0) {
                 if (v < 256) atomicAdd(&hist[v], 1u);
             } else if (len >= 3 && len <= 258 && v >= 1 && v <= 32768) {
@@ -1381,7 +1415,7 @@ void launch_match_find(const Work &w, uint32_t grid, int summary, hipStream_t st
     hipLaunchKernelGGL(k_match_find, dim3(grid), dim3(K1_T), 0, st, w, summary);
 }
 void launch_walk_summary(const Work &w, uint32_t grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_walk_summary, dim3(grid), dim3(K1_T), 0, st, w);
+    hipLaunchKernelGGL(k_walk_summary, dim3(grid), dim3(KW_T), 0, st, w);
 }
 void launch_chain(const Work &w, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_chain, dim3(grid), dim3(64), 0, st, w);
