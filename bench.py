@@ -217,6 +217,25 @@ def cpu_baselines(data: np.ndarray, sample_bytes: int, chunk: int):
     return port, zl
 
 
+def profiled_traffic(n_in: int):
+    """HBM bytes per k_match_find launch as last measured with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by a copy-kernel calibration);
+    the counters cannot be read from inside this process.  None if the
+    committed profile is for another input size."""
+    path = os.path.join(ROOT, "profiles", "k_match_find_binmix256_traffic.md")
+    vals = {}
+    try:
+        for line in open(path):
+            k, _, v = line.partition(":")
+            if k.strip() in ("traffic_bytes_per_launch", "input_bytes"):
+                vals[k.strip()] = int(v.strip())
+    except OSError:
+        return None
+    if vals.get("input_bytes") != n_in or "traffic_bytes_per_launch" not in vals:
+        return None
+    return vals["traffic_bytes_per_launch"]
+
+
 def mixed_sizes_workload(seed=3, count=4096):
     sizes = gpzip_amd.loguniform_sizes(seed, count, 4096, 4 << 20).astype(np.int64)
     choose = gpzip_amd.uniform(seed + 1, count)
@@ -287,6 +306,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-load", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain elementwise copy (y = x + 1, uint8) of MIB MiB three times; "
+                         "profiling its FETCH_SIZE / WRITE_SIZE gives the counter-to-bytes factor")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested)",
@@ -294,6 +316,17 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.calibrate_copy:
+        nb = args.calibrate_copy * MiB
+        x = torch.zeros(nb, dtype=torch.uint8, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.add(x, 1, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "elementwise uint8 copy y = x + 1", "read_bytes": nb,
+                          "write_bytes": nb, "launches": 3}), flush=True)
+        return
 
     log("generating binmix 256 MiB ...")
     n = 256 * MiB
@@ -330,14 +363,15 @@ def main():
                     "vs_plain": round(r["batched"]["gbps"] / r["plain"]["gbps"], 3)},
         "roofline": {"bound": "hbm", "kernel": "k_match_find", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
-                     "traffic": None, "alg_bytes_per_launch": alg_bytes,
+                     "traffic": profiled_traffic(r["n_in"]), "alg_bytes_per_launch": alg_bytes,
+                     "traffic_source": "profiles/k_match_find_binmix256_traffic.md (rocprofv3 --pmc)",
                      "avg_launch_ms": round(r["k1_ms"], 4), "seg_table_ms": round(r["k0_ms"], 4),
                      "emit_ms": round(r["emit_ms"], 4),
                      "pipeline_ms": round(r["pipeline_ms"], 4)},
     }
     if not args.no_cpu:
         log("cpu baselines ...")
-        port, zl = cpu_baselines(data, 32 * MiB, 2 * MiB)
+        port, zl = cpu_baselines(data, 64 * MiB, 2 * MiB)
         line["cpu_baseline"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in port.items()}
         line["cpu_zlib"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in zl.items()}
     print(json.dumps(line), flush=True)

@@ -118,6 +118,40 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *tmp, uint32
     return base + x - v;
 }
 
+// Inclusive scan over the 64 lanes of a wave with DPP row shifts and row
+// broadcasts (no lane-index registers, unlike a shuffle-based scan): rows of
+// 16 lanes first, then lane 15 into row 1 and row 3, then lane 31 into rows 2
+// and 3.  Lanes whose source is outside the row read the `old` operand, 0.
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
+}
+
+// Workgroup exclusive scan built on wave_scan_incl; tmp holds BLOCK / 64
+// words.  Ends with a barrier, so tmp may be reused at once.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_scan_dpp(uint32_t v, uint32_t *tmp, uint32_t *total) {
+    const uint32_t wid = threadIdx.x >> 6;
+    const uint32_t inc = wave_scan_incl(v);
+    if ((threadIdx.x & 63) == 63) tmp[wid] = inc;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < BLOCK / 64; j++) {
+        const uint32_t t = tmp[j];
+        before += j < wid ? t : 0;
+        tot += t;
+    }
+    *total = tot;
+    __syncthreads();
+    return before + inc - v;
+}
+
 template <int BLOCK>
 __device__ __forceinline__ uint64_t block_scan64(uint64_t v, uint64_t *tmp, uint64_t *total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -143,6 +177,25 @@ __device__ __forceinline__ uint64_t block_scan64(uint64_t v, uint64_t *tmp, uint
     *total = tmp[BLOCK / 64 - 1];
     __syncthreads();
     return base + x - v;
+}
+
+// Values that are the same on every lane (workgroup-uniform loads) moved to
+// scalar registers.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v);
+}
+// Member descriptor m, read by a whole workgroup (m must be uniform).
+__device__ __forceinline__ MemberDesc load_member(const MemberDesc *mem, uint32_t m) {
+    const MemberDesc d = mem[m];
+    MemberDesc r;
+    r.in_off = uni64(d.in_off);
+    r.pos_base = uni64(d.pos_base);
+    r.n = uni(d.n);
+    r.seg_base = uni(d.seg_base);
+    r.nseg = uni(d.nseg);
+    r.chunk = uni(d.chunk);
+    return r;
 }
 
 // Spread consecutive work items of a persistent grid so that each XCD (blocks
@@ -242,10 +295,10 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
     __shared__ uint32_t crc_tab[256];
     __shared__ uint32_t tmp[16];
     const uint32_t tid = threadIdx.x;
-    const uint32_t G = w.counts->nseg;
+    const uint32_t G = uni(w.counts->nseg);
     for (uint32_t i = tid; i < 256; i += K0_T) crc_tab[i] = c_crc_tab[i];
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
-        const MemberDesc md = w.mem[w.seg_mem[g]];
+        const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
         // bytes x0 .. x0+11 of the segment (buffers are padded past every member)
@@ -352,77 +405,61 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
 // by rank in a 65536-bit presence map.  Then for k = 0..8 (the reference's
 // segment order, lz77_hash_lookup.wgsl:85-112; older segments only give
 // distances > 32768, rejected at :50) the compact table of segment s-k is
-// streamed in -- prefetched into registers one step ahead -- and every entry
-// whose hash occurs in the segment stores its (first, last) under the id,
-// tagged with k; each position then tries newest, then oldest while
-// best < 16, stopping at 258 (:96-110).  65 KB of LDS: two workgroups per CU.
+// streamed in and every entry whose hash occurs in the segment stores its
+// (first, last) under the id, tagged with k; each position then tries newest,
+// then oldest while best < 16, stopping at 258 (:96-110).  65 KB of LDS: two
+// workgroups per CU.
 
 constexpr int K1_T = 1024;
 constexpr int WIN_WORDS = 9344;  // 37,376 bytes: 32 KiB back + 4 KiB segment + 258 + slack
-constexpr int KSLOTS = 8192;
 
-struct MatchState {
-    uint32_t len, dist;
-};
-
-// try_candidate (lz77_hash_lookup.wgsl:45-67) on the LDS window.  `d0` is the
-// xor of the first four bytes at p and cand; `known` (>= 3 or 0) is a length
-// over which (p, cand) is already known to agree.  st = best len << 16 | dist.
-__device__ __forceinline__ uint32_t try_cand(const uint32_t *win, uint32_t WB, uint32_t n, uint32_t p,
-                                             uint32_t cand, uint32_t window, uint32_t d0, uint32_t known,
-                                             uint32_t st) {
-    if (cand >= p) return st;
-    const uint32_t dist = p - cand;
-    if (dist > window) return st;
-    const uint32_t xp = p - WB, xc = cand - WB;
+// Length of the match at cand (0 if the reference rejects it: cand >= p,
+// dist > window, or a mismatch in the first three bytes), capped at
+// min(258, n - p).  `own4` holds the first four bytes at p; `known` (>= 3 or
+// 0) is a length over which (p, cand) is already known to agree, from the
+// previous position at the same distance.
+__device__ __forceinline__ uint32_t cand_len(const uint32_t *win, uint32_t WB, uint32_t n, uint32_t p,
+                                             uint32_t cand, uint32_t window, uint32_t own4, uint32_t known) {
+    if (cand >= p || p - cand > window) return 0;
     uint32_t lim = n - p;
     if (lim > MAX_MATCH) lim = MAX_MATCH;
     uint32_t len = known;
-    bool extend = true;
     if (!known) {
-        if (d0 & 0x00FFFFFFu) return st;
-        len = d0 ? 3 : 4;
-        extend = d0 == 0;
+        const uint32_t d0 = own4 ^ ld4(win, cand - WB);
+        if (d0 & 0x00FFFFFFu) return 0;
+        if (d0) return 3;
+        len = 4;
     }
-    if (extend) {
-        // 8 bytes per step from aligned 64-bit LDS words
-        const uint64_t *w64 = reinterpret_cast<const uint64_t *>(win);
-        const uint32_t ap = xp + len, ac = xc + len;
-        uint32_t qa = ap >> 3, qc = ac >> 3;
-        const uint32_t sa = (ap & 7u) * 8, sc = (ac & 7u) * 8;
-        uint64_t a0 = w64[qa], c0 = w64[qc];
-        while (len < lim) {
-            const uint64_t a1 = w64[++qa], c1 = w64[++qc];
-            const uint64_t av = sa ? (a0 >> sa) | (a1 << (64 - sa)) : a0;
-            const uint64_t cv = sc ? (c0 >> sc) | (c1 << (64 - sc)) : c0;
-            const uint64_t x = av ^ cv;
-            if (x) {
-                len += (uint32_t)__builtin_ctzll(x) >> 3;
-                break;
-            }
-            len += 8;
-            a0 = a1;
-            c0 = c1;
+    // 8 bytes per step from aligned 64-bit LDS words
+    const uint64_t *w64 = reinterpret_cast<const uint64_t *>(win);
+    const uint32_t ap = p - WB + len, ac = cand - WB + len;
+    uint32_t qa = ap >> 3, qc = ac >> 3;
+    const uint32_t sa = (ap & 7u) * 8, sc = (ac & 7u) * 8;
+    uint64_t a0 = w64[qa], c0 = w64[qc];
+    while (len < lim) {
+        const uint64_t a1 = w64[++qa], c1 = w64[++qc];
+        const uint64_t av = sa ? (a0 >> sa) | (a1 << (64 - sa)) : a0;
+        const uint64_t cv = sc ? (c0 >> sc) | (c1 << (64 - sc)) : c0;
+        const uint64_t x = av ^ cv;
+        if (x) {
+            len += (uint32_t)__builtin_ctzll(x) >> 3;
+            break;
         }
+        len += 8;
+        a0 = a1;
+        c0 = c1;
     }
-    if (len > lim) len = lim;
-    return len > (st >> 16) ? (len << 16) | dist : st;
+    return len > lim ? lim : len;
 }
 
-// Entries of a segment's compact table handled by one thread (D <= 4096).
-struct Ent4 {
-    uint2 e[4];
-    uint32_t D;
-};
-
-__device__ __forceinline__ void load_ent(const uint2 *tabs, const uint32_t *ntab, uint32_t gj, uint32_t tid, Ent4 &o) {
-    o.D = ntab[gj];
-    const uint2 *T = tabs + (uint64_t)gj * SEG;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const uint32_t i = tid + r * K1_T;
-        o.e[r] = i < o.D ? T[i] : make_uint2(0, 0);
-    }
+// Known agreement of (p, p - d) from results at p - 1 (len << 16 | dist): a
+// match of length m at p - 1 and distance d means m - 1 equal bytes at p.
+__device__ __forceinline__ uint32_t known_from(uint32_t d, uint32_t h0, uint32_t h1, uint32_t h2) {
+    uint32_t m = 0;
+    if ((h0 & 0xFFFFu) == d) m = h0 >> 16;
+    if ((h1 & 0xFFFFu) == d) m = max(m, h1 >> 16);
+    if ((h2 & 0xFFFFu) == d) m = max(m, h2 >> 16);
+    return m >= 4 ? m - 1 : 0;
 }
 
 // The distinct hashes of the query segment get dense ids by rank: a 65536-bit
@@ -439,66 +476,65 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
     __shared__ uint32_t fl[SEG];     // per hash id: tag << 24 | first << 12 | last
     __shared__ uint32_t tmp[32];
     const uint32_t tid = threadIdx.x;
-    const uint32_t G = w.counts->nseg;
+    const uint32_t G = uni(w.counts->nseg);
     const uint2 *tabs = reinterpret_cast<const uint2 *>(w.segtab);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
-        const MemberDesc md = w.mem[w.seg_mem[g]];
+        const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2;
         const uint32_t E = min(S + SEG, n), L = E - S;
         const uint32_t WB = S >= 32768u ? S - 32768u : 0u;
         const uint32_t WE = min(n, E + 272u);
         const uint32_t nb = WE - WB;
-        const uint8_t *src = w.in + md.in_off + WB;
         const uint32_t kmax = min(s, 8u);
-        Ent4 own;
-        load_ent(tabs, w.seg_ntab, g, tid, own);
 
         // ---- window -> LDS (zero beyond the member end)
-        for (uint32_t v = tid; v < WIN_WORDS / 4; v += K1_T) {
-            const uint32_t b0 = v * 16u;
-            uint4 q = make_uint4(0, 0, 0, 0);
-            if (b0 < nb) {
-                q = *reinterpret_cast<const uint4 *>(src + b0);
-                if (b0 + 16u > nb) {
-                    const uint32_t rem = nb - b0;
-                    uint32_t *qq = reinterpret_cast<uint32_t *>(&q);
+        {
+            const uint8_t *src = w.in + md.in_off + WB;
+            for (uint32_t v = tid; v < WIN_WORDS / 4; v += K1_T) {
+                const uint32_t b0 = v * 16u;
+                uint4 q = make_uint4(0, 0, 0, 0);
+                if (b0 < nb) {
+                    q = *reinterpret_cast<const uint4 *>(src + b0);
+                    if (b0 + 16u > nb) {
+                        const uint32_t rem = nb - b0;
+                        uint32_t *qq = reinterpret_cast<uint32_t *>(&q);
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const uint32_t lo = 4u * j;
-                        if (lo >= rem) qq[j] = 0;
-                        else if (lo + 4 > rem) qq[j] &= (1u << (8 * (rem - lo))) - 1u;
+                        for (int j = 0; j < 4; j++) {
+                            const uint32_t lo = 4u * j;
+                            if (lo >= rem) qq[j] = 0;
+                            else if (lo + 4 > rem) qq[j] &= (1u << (8 * (rem - lo))) - 1u;
+                        }
                     }
                 }
+                reinterpret_cast<uint4 *>(win)[v] = q;
             }
-            reinterpret_cast<uint4 *>(win)[v] = q;
         }
         for (uint32_t i = tid; i < 2048; i += K1_T) bits[i] = 0;
         __syncthreads();
 
         // ---- presence map of the segment's own hashes, then ranks
+        {
+            const uint32_t D = uni(w.seg_ntab[g]);
+            const uint2 *T = tabs + (uint64_t)g * SEG;
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-            if (tid + r * K1_T < own.D) {
-                const uint32_t h = own.e[r].x & 0xFFFFu;
-                atomicOr(&bits[h >> 5], 1u << (h & 31));
+            for (int r = 0; r < 4; r++) {
+                const uint32_t i = tid + r * K1_T;
+                if (i < D) {
+                    const uint32_t h = T[i].x & 0xFFFFu;
+                    atomicOr(&bits[h >> 5], 1u << (h & 31));
+                }
             }
+        }
         __syncthreads();
         {
             const uint32_t b0 = bits[2 * tid], b1 = bits[2 * tid + 1];
             uint32_t tot;
-            const uint32_t ex = block_scan<K1_T>(__popc(b0) + __popc(b1), tmp, &tot);
+            const uint32_t ex = block_scan_dpp<K1_T>(__popc(b0) + __popc(b1), tmp, &tot);
             rnk[2 * tid] = (uint16_t)ex;
             rnk[2 * tid + 1] = (uint16_t)(ex + __popc(b0));
         }
         __syncthreads();
-        // k = 0 candidates straight from the own table
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-            if (tid + r * K1_T < own.D) {
-                const uint2 e = own.e[r];
-                fl[hash_rank(bits, rnk, e.x & 0xFFFFu)] = ((e.x >> 16) << 12) | e.y;  // tag 0
-            }
 
         // ---- own positions: hash -> id
         const uint32_t x0 = tid * 4u;
@@ -517,58 +553,63 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
                 hvm |= (uint32_t)hv << i;
             }
         }
-        __syncthreads();
 
         uint32_t st[4] = {0, 0, 0, 0};  // best len << 16 | dist per position
         for (uint32_t k = 0; k <= kmax; k++) {
-            if (k) {
-                // stream segment s-k's entries
-                Ent4 now;
-                load_ent(tabs, w.seg_ntab, g - k, tid, now);
+            // stream segment s-k's compact table: every entry whose hash occurs
+            // in this segment stores its (first, last) under the hash's id
+            {
+                const uint32_t gk = g - k;
+                const uint32_t D = uni(w.seg_ntab[gk]);
+                const uint2 *T = tabs + (uint64_t)gk * SEG;
                 const uint32_t tag = k << 24;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    if (tid + r * K1_T >= now.D) continue;
-                    const uint2 e = now.e[r];
+                    const uint32_t i = tid + r * K1_T;
+                    if (i >= D) continue;
+                    const uint2 e = T[i];
                     const uint32_t h = e.x & 0xFFFFu;
                     if (has_hash(bits, h)) fl[hash_rank(bits, rnk, h)] = tag | ((e.x >> 16) << 12) | e.y;
                 }
-                __syncthreads();
             }
+            __syncthreads();
             // evaluate segment s-k: gather both candidates' first bytes for all
             // four positions, then apply the reference order position by
-            // position (arrays rotate so the loop body is not unrolled)
+            // position (arrays rotate so the loop body is not unrolled).  Each
+            // candidate's length at the previous position seeds the next one.
             const uint32_t base = (s - k) << SEG_LOG2;
-            uint32_t fv[4], wn[4], wo[4];
+            uint32_t fv[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t v = (((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH) ? fl[qid[i]] : EMPTY;
-                fv[i] = v;
-                const bool ok = (v >> 24) == k;
-                const uint32_t p = S + x0 + i;
-                const uint32_t cn = base + (v & 0xFFFu), co = base + ((v >> 12) & 0xFFFu);
-                wn[i] = (ok && cn < p && p - cn <= w.window) ? ld4(win, cn - WB) : 0;
-                wo[i] = (ok && co < p && p - co <= w.window) ? ld4(win, co - WB) : 0;
-            }
-            uint32_t prev = 0;  // state of the previous position (prefix reuse)
+            for (int i = 0; i < 4; i++)
+                fv[i] = (((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH) ? fl[qid[i]] : EMPTY;
+            uint32_t hb = 0, hn = 0, ho = 0;  // previous position: best, newest, oldest (len << 16 | dist)
 #pragma unroll 1
             for (uint32_t i = 0; i < 4; i++) {
                 const uint32_t v = fv[0];
                 uint32_t cs = st[0];
+                uint32_t rn = 0, ro = 0;
                 if ((v >> 24) == k) {
                     const uint32_t p = S + x0 + i;
                     const uint32_t own4 = ld4(win, lx + i);
                     const uint32_t cn = base + (v & 0xFFFu), co = base + ((v >> 12) & 0xFFFu);
-                    const uint32_t hd = prev & 0xFFFFu;
-                    const uint32_t hk = (prev >> 16) >= 4 ? (prev >> 16) - 1 : 0;
-                    cs = try_cand(win, WB, n, p, cn, w.window, own4 ^ wn[0], (p - cn == hd) ? hk : 0, cs);
-                    if ((cs >> 16) < OLDEST_SKIP)
-                        cs = try_cand(win, WB, n, p, co, w.window, own4 ^ wo[0], (p - co == hd) ? hk : 0, cs);
+                    const uint32_t dn = p - cn, dd = p - co;
+                    const uint32_t ln = cand_len(win, WB, n, p, cn, w.window, own4, known_from(dn, hb, hn, ho));
+                    rn = (ln << 16) | (dn & 0xFFFFu);
+                    if (ln > (cs >> 16)) cs = (ln << 16) | dn;
+                    // the oldest entry is tried only while best < 16
+                    // (lz77_hash_lookup.wgsl:102); equal to the newest it cannot
+                    // give a strictly longer match
+                    if ((cs >> 16) < OLDEST_SKIP && co != cn) {
+                        const uint32_t lo =
+                            cand_len(win, WB, n, p, co, w.window, own4, known_from(dd, hb, hn, ho));
+                        ro = (lo << 16) | (dd & 0xFFFFu);
+                        if (lo > (cs >> 16)) cs = (lo << 16) | dd;
+                    }
                 }
-                prev = cs;
+                hb = cs;
+                hn = rn;
+                ho = ro;
                 fv[0] = fv[1]; fv[1] = fv[2]; fv[2] = fv[3];
-                wn[0] = wn[1]; wn[1] = wn[2]; wn[2] = wn[3];
-                wo[0] = wo[1]; wo[1] = wo[2]; wo[2] = wo[3];
                 st[0] = st[1]; st[1] = st[2]; st[2] = st[3]; st[3] = cs;
             }
             int active = 0;
@@ -593,6 +634,7 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
                 if (x0 + i < L) tg[i] = t4[i];
         }
         (void)do_summary;
+        __syncthreads();
     }
 }
 
@@ -607,9 +649,9 @@ __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
     __shared__ uint32_t tokL[SEG];
     __shared__ uint16_t Ea[SEG], Ca[SEG];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const uint32_t G = w.counts->nseg;
+    const uint32_t G = uni(w.counts->nseg);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
-        const MemberDesc md = w.mem[w.seg_mem[g]];
+        const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
         const bool last_seg = E == n;
@@ -665,10 +707,10 @@ __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
 __global__ __launch_bounds__(64) void k_chain(Work w) {
     __shared__ uint32_t sum[32 * SPAN];
     __shared__ uint32_t btok[64];
-    const uint32_t M = w.counts->nmem;
+    const uint32_t M = uni(w.counts->nmem);
     const uint32_t lane = threadIdx.x;
     for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
-        const MemberDesc md = w.mem[m];
+        const MemberDesc md = load_member(w.mem, m);
         const uint32_t nseg = md.nseg;
         const bool cached = nseg <= 32;
         if (cached) {
@@ -741,10 +783,10 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
     __shared__ uint32_t hist[NSYM];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint64_t lt = (1ull << lane) - 1;
-    const uint32_t G = w.counts->nseg;
+    const uint32_t G = uni(w.counts->nseg);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
-        const uint32_t m = w.seg_mem[g];
-        const MemberDesc md = w.mem[m];
+        const uint32_t m = uni(w.seg_mem[g]);
+        const MemberDesc md = load_member(w.mem, m);
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
         const uint32_t e = w.seg_entry[g];
@@ -847,9 +889,9 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
 // Histogram of caller-supplied walked tokens (stage-level gpz_encode_block).
 __global__ __launch_bounds__(1024) void k_hist_tokens(Work w) {
     __shared__ uint32_t hist[NSYM];
-    const uint32_t M = w.counts->nmem;
+    const uint32_t M = uni(w.counts->nmem);
     for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
-        const MemberDesc md = w.mem[m];
+        const MemberDesc md = load_member(w.mem, m);
         for (uint32_t i = threadIdx.x; i < NSYM; i += 1024) hist[i] = 0;
         __syncthreads();
         const uint32_t nt = w.ntok[m];
@@ -1049,10 +1091,10 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
     __shared__ uint32_t pbuf[PREFIX_WORDS + 2];
     __shared__ uint32_t info[8];
     __shared__ HuffScratch hs;
-    const uint32_t M = w.counts->nmem;
+    const uint32_t M = uni(w.counts->nmem);
     const int lane = threadIdx.x;
     for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
-        const MemberDesc md = w.mem[m];
+        const MemberDesc md = load_member(w.mem, m);
         for (int i = lane; i < (int)NSYM; i += 64) freq[i] = w.hist[(uint64_t)m * NSYM + i];
         for (int i = lane; i < 20; i += 64) mfreq[i] = 0;
         __syncthreads();
@@ -1206,7 +1248,7 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
 
 __global__ __launch_bounds__(1024) void k_scan(Work w) {
     __shared__ uint64_t tmp[16];
-    const uint32_t M = w.counts->nmem, C = w.counts->nchunk;
+    const uint32_t M = uni(w.counts->nmem), C = uni(w.counts->nchunk);
     uint64_t carry = 0;
     for (uint32_t b = 0; b < M; b += 1024) {
         uint32_t i = b + threadIdx.x;
@@ -1281,9 +1323,9 @@ __global__ __launch_bounds__(K6_T) void k_emit(Work w) {
     __shared__ uint32_t buf[BUF_WORDS];
     __shared__ uint32_t tmp[32];
     const uint32_t tid = threadIdx.x;
-    const uint32_t M = w.counts->nmem;
+    const uint32_t M = uni(w.counts->nmem);
     for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
-        const MemberDesc md = w.mem[m];
+        const MemberDesc md = load_member(w.mem, m);
         for (uint32_t i = tid; i < NSYM; i += K6_T) ctab[i] = w.codes[(uint64_t)m * NSYM + i];
         Emitter em;
         em.buf = buf;
