@@ -430,24 +430,25 @@ __device__ __forceinline__ uint32_t cand_len(const uint32_t *win, uint32_t WB, u
         if (d0) return 3;
         len = 4;
     }
-    // 8 bytes per step from aligned 64-bit LDS words
-    const uint64_t *w64 = reinterpret_cast<const uint64_t *>(win);
+    // 8 bytes per step: dwords of both sides realigned with alignbyte; one
+    // exit (mismatch found or the cap reached)
     const uint32_t ap = p - WB + len, ac = cand - WB + len;
-    uint32_t qa = ap >> 3, qc = ac >> 3;
-    const uint32_t sa = (ap & 7u) * 8, sc = (ac & 7u) * 8;
-    uint64_t a0 = w64[qa], c0 = w64[qc];
-    while (len < lim) {
-        const uint64_t a1 = w64[++qa], c1 = w64[++qc];
-        const uint64_t av = sa ? (a0 >> sa) | (a1 << (64 - sa)) : a0;
-        const uint64_t cv = sc ? (c0 >> sc) | (c1 << (64 - sc)) : c0;
-        const uint64_t x = av ^ cv;
-        if (x) {
-            len += (uint32_t)__builtin_ctzll(x) >> 3;
+    const uint32_t sa = ap & 3u, sc = ac & 3u;
+    uint32_t ia = ap >> 2, ic = ac >> 2;
+    uint32_t a0 = win[ia], c0 = win[ic];
+    for (;;) {
+        const uint32_t a1 = win[ia + 1], a2 = win[ia + 2], c1 = win[ic + 1], c2 = win[ic + 2];
+        const uint32_t x0 = __builtin_amdgcn_alignbyte(a1, a0, sa) ^ __builtin_amdgcn_alignbyte(c1, c0, sc);
+        const uint32_t x1 = __builtin_amdgcn_alignbyte(a2, a1, sa) ^ __builtin_amdgcn_alignbyte(c2, c1, sc);
+        if ((x0 | x1) != 0 || len + 8 >= lim) {
+            len += x0 ? (uint32_t)__builtin_ctz(x0) >> 3 : (x1 ? 4 + ((uint32_t)__builtin_ctz(x1) >> 3) : 8);
             break;
         }
         len += 8;
-        a0 = a1;
-        c0 = c1;
+        ia += 2;
+        ic += 2;
+        a0 = a2;
+        c0 = c2;
     }
     return len > lim ? lim : len;
 }
@@ -541,8 +542,9 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
         const uint32_t lx = S - WB + x0;
         uint32_t qid[4];
         uint32_t hvm = 0;  // bit i: position x0+i can start a match (p + 2 < n)
+        const uint32_t own_lo = win[lx >> 2], own_hi = win[(lx >> 2) + 1];  // bytes x0 .. x0+7
         {
-            const uint64_t vv = (uint64_t)win[lx >> 2] | ((uint64_t)win[(lx >> 2) + 1] << 32);
+            const uint64_t vv = (uint64_t)own_lo | ((uint64_t)own_hi << 32);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint32_t x = x0 + i;
@@ -590,7 +592,7 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
                 uint32_t rn = 0, ro = 0;
                 if ((v >> 24) == k) {
                     const uint32_t p = S + x0 + i;
-                    const uint32_t own4 = ld4(win, lx + i);
+                    const uint32_t own4 = __builtin_amdgcn_alignbyte(own_hi, own_lo, i);
                     const uint32_t cn = base + (v & 0xFFFu), co = base + ((v >> 12) & 0xFFFu);
                     const uint32_t dn = p - cn, dd = p - co;
                     const uint32_t ln = cand_len(win, WB, n, p, cn, w.window, own4, known_from(dn, hb, hn, ho));
@@ -621,9 +623,8 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
         // ---- tokens
         uint32_t t4[4];
         {
-            const uint32_t ob = ld4(win, lx);
 #pragma unroll
-            for (int i = 0; i < 4; i++) t4[i] = st[i] ? st[i] : (ob >> (8 * i)) & 0xFF;
+            for (int i = 0; i < 4; i++) t4[i] = st[i] ? st[i] : (own_lo >> (8 * i)) & 0xFF;
         }
         uint32_t *tg = w.tok + md.pos_base + S + x0;
         if (x0 + 4 <= L) {
