@@ -21,6 +21,11 @@ constexpr uint32_t DSYM0 = 286;        // first dist slot in the 320-entry table
 constexpr uint32_t PREFIX_WORDS = 160; // gzip header + dynamic block header bits
 constexpr uint32_t POS_ALIGN = 64;     // member token regions start on 64-position boundaries
 constexpr uint32_t IN_ALIGN = 256;     // chunk data alignment in device input buffers
+// Per-segment hash table (k_seg_table -> k_match_find), 7168 words:
+//   [0, 2048)      presence bitmap of the segment's 16-bit hashes
+//   [2048, 3072)   2048 x u16: set bits before each bitmap word (rank)
+//   [3072, 7168)   per hash, in hash order: first << 12 | last position
+constexpr uint32_t TAB_BITS = 0, TAB_RANK = 2048, TAB_ENT = 3072, TAB_WORDS = 7168;
 
 // One gzip member: n input bytes at in_off in the device input buffer.
 struct MemberDesc {
@@ -45,8 +50,8 @@ struct Work {
     const uint32_t *chunk_mem0; // [C] first member of each chunk
     const Counts *counts;
     uint32_t *tok;              // [P] per-position tokens
-    uint64_t *segtab;           // [G * 4096] per-segment compact (hash|first<<16, last) lists
-    uint32_t *seg_ntab;         // [G] entries in each list
+    uint32_t *segtab;           // [G * TAB_WORDS] per-segment hash tables (layout above)
+    uint32_t *seg_ntab;         // [G] distinct hashes of each segment
     uint32_t *wsum;             // [G * SPAN] walk summary: exit | count << 16
     uint32_t *seg_crc;          // [G] raw CRC-32 (init 0, no final xor) of the segment
     uint32_t *seg_entry;        // [G] walk entry offset into the segment

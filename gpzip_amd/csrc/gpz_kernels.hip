@@ -281,7 +281,6 @@ __device__ __forceinline__ uint32_t block_walk(uint32_t J, uint32_t bs, uint32_t
 // window covers it.  Also computes the segment's raw CRC-32.
 
 constexpr int K0_T = 512;
-constexpr uint32_t TSLOTS = 8192;
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
 // hash3 of lz77_hash_build.wgsl:36-43 on three bytes
@@ -289,9 +288,16 @@ __device__ __forceinline__ uint32_t hash3(uint32_t b0, uint32_t b1, uint32_t b2)
     return ((b0 << 16 | b1 << 8 | b2) * 0x9E3779B1u) >> 16;
 }
 
+// k_seg_table: the reference's build pass (lz77_hash_build.wgsl:36-54,
+// oldest/newest p+1 per (hash, segment) by global atomics) restated per
+// segment in LDS: the set of 16-bit hashes of the segment as a bitmap, their
+// ranks, and per hash (in hash order) its first and last position.  Also the
+// raw CRC-32 of the segment's bytes.
 __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
-    __shared__ uint32_t keys[TSLOTS];
-    __shared__ uint32_t acc[TSLOTS];
+    __shared__ uint32_t bm[2048];
+    __shared__ uint16_t rk[2048];
+    __shared__ uint32_t fst[SEG];
+    __shared__ uint32_t lst[SEG];
     __shared__ uint32_t crc_tab[256];
     __shared__ uint32_t tmp[16];
     const uint32_t tid = threadIdx.x;
@@ -301,6 +307,7 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
         const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
+        uint32_t *T = w.segtab + (uint64_t)g * TAB_WORDS;
         // bytes x0 .. x0+11 of the segment (buffers are padded past every member)
         const uint32_t x0 = tid * 8;
         const uint32_t *src = reinterpret_cast<const uint32_t *>(w.in + md.in_off + S + x0);
@@ -311,61 +318,54 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
             d2 = src[2];
         }
         const uint64_t lo = (uint64_t)d0 | ((uint64_t)d1 << 32);
-        for (uint32_t i = tid; i < TSLOTS; i += K0_T) {
-            keys[i] = EMPTY;
-            acc[i] = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) bm[tid + r * K0_T] = 0;
+#pragma unroll
+        for (int r = 0; r < (int)(SEG / K0_T); r++) {
+            fst[tid + r * K0_T] = EMPTY;
+            lst[tid + r * K0_T] = 0;
         }
         __syncthreads();
-        uint32_t slot[8];
-        bool hv[8];
+        uint32_t hh[8];
+        uint32_t hvm = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const uint32_t x = x0 + i;
-            hv[i] = x < L && S + x + 2 < n;
-            uint32_t b0 = (uint32_t)(lo >> (8 * i)) & 0xFF;
-            uint32_t b1 = i + 1 < 8 ? (uint32_t)(lo >> (8 * (i + 1))) & 0xFF : d2 & 0xFF;
-            uint32_t b2 = i + 2 < 8 ? (uint32_t)(lo >> (8 * (i + 2))) & 0xFF : (d2 >> (8 * (i + 2 - 8))) & 0xFF;
-            uint32_t h = hash3(b0, b1, b2);
-            slot[i] = 0;
-            if (hv[i]) {
-                uint32_t sl = h & (TSLOTS - 1);
-                for (;;) {
-                    uint32_t prev = atomicCAS(&keys[sl], EMPTY, h);
-                    if (prev == EMPTY || prev == h) break;
-                    sl = (sl + 1) & (TSLOTS - 1);
-                }
-                slot[i] = sl;
-                atomicMax(&acc[sl], x + 1);
-            }
+            const bool hv = x < L && S + x + 2 < n;
+            const uint32_t b0 = (uint32_t)(lo >> (8 * i)) & 0xFF;
+            const uint32_t b1 = i + 1 < 8 ? (uint32_t)(lo >> (8 * (i + 1))) & 0xFF : d2 & 0xFF;
+            const uint32_t b2 = i + 2 < 8 ? (uint32_t)(lo >> (8 * (i + 2))) & 0xFF : (d2 >> (8 * (i + 2 - 8))) & 0xFF;
+            hh[i] = hash3(b0, b1, b2);
+            hvm |= (uint32_t)hv << i;
+            if (hv) atomicOr(&bm[hh[i] >> 5], 1u << (hh[i] & 31));
         }
         __syncthreads();
-        uint32_t last[TSLOTS / K0_T];
-#pragma unroll
-        for (int r = 0; r < (int)(TSLOTS / K0_T); r++) last[r] = acc[tid + r * K0_T];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < (int)(TSLOTS / K0_T); r++) acc[tid + r * K0_T] = EMPTY;
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 8; i++)
-            if (hv[i]) atomicMin(&acc[slot[i]], x0 + i);
-        __syncthreads();
+        uint32_t D;  // distinct hashes in the segment
         {
-            // compact the distinct hashes of the segment: (hash | first << 16, last)
-            uint32_t cnt = 0;
-#pragma unroll
-            for (int r = 0; r < (int)(TSLOTS / K0_T); r++) cnt += keys[tid + r * K0_T] != EMPTY;
-            uint32_t total;
-            uint32_t idx = block_scan<K0_T>(cnt, tmp, &total);
-            uint2 *tab = reinterpret_cast<uint2 *>(w.segtab) + (uint64_t)g * SEG;
-#pragma unroll
-            for (int r = 0; r < (int)(TSLOTS / K0_T); r++) {
-                const uint32_t sl = tid + r * K0_T;
-                const uint32_t k = keys[sl];
-                if (k != EMPTY) tab[idx++] = make_uint2(k | (acc[sl] << 16), last[r] - 1);
-            }
-            if (tid == 0) w.seg_ntab[g] = total;
+            // ranks of words 4t .. 4t+3; bitmap and ranks go out as they are
+            const uint4 q = reinterpret_cast<const uint4 *>(bm)[tid];
+            const uint32_t c0 = __popc(q.x), c1 = __popc(q.y), c2 = __popc(q.z);
+            const uint32_t ex = block_scan_dpp<K0_T>(c0 + c1 + c2 + __popc(q.w), tmp, &D);
+            const uint32_t r0 = ex, r1 = ex + c0, r2 = r1 + c1, r3 = r2 + c2;
+            rk[4 * tid] = (uint16_t)r0;
+            rk[4 * tid + 1] = (uint16_t)r1;
+            rk[4 * tid + 2] = (uint16_t)r2;
+            rk[4 * tid + 3] = (uint16_t)r3;
+            reinterpret_cast<uint4 *>(T + TAB_BITS)[tid] = q;
+            reinterpret_cast<uint2 *>(T + TAB_RANK)[tid] = make_uint2(r0 | (r1 << 16), r2 | (r3 << 16));
+            if (tid == 0) w.seg_ntab[g] = D;
         }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            if (!((hvm >> i) & 1u)) continue;
+            const uint32_t h = hh[i];
+            const uint32_t id = rk[h >> 5] + __popc(bm[h >> 5] & ((1u << (h & 31)) - 1u));
+            atomicMin(&fst[id], x0 + i);
+            atomicMax(&lst[id], x0 + i);
+        }
+        __syncthreads();
+        for (uint32_t id = tid; id < D; id += K0_T) T[TAB_ENT + id] = (fst[id] << 12) | lst[id];
         // raw CRC-32 of the segment: 8 bytes per thread, folded by x^(8 * bytes after them)
         {
             uint32_t c = 0;
@@ -414,17 +414,16 @@ constexpr int K1_T = 1024;
 constexpr int WIN_WORDS = 9344;  // 37,376 bytes: 32 KiB back + 4 KiB segment + 258 + slack
 
 // Length of the match at cand (0 if the reference rejects it: cand >= p,
-// dist > window, or a mismatch in the first three bytes), capped at
-// min(258, n - p).  `own4` holds the first four bytes at p; `known` (>= 3 or
-// 0) is a length over which (p, cand) is already known to agree, from the
-// previous position at the same distance.
-__device__ __forceinline__ uint32_t cand_len(const uint32_t *win, uint32_t WB, uint32_t n, uint32_t p,
-                                             uint32_t cand, uint32_t window, uint32_t own4, uint32_t known) {
+// dist > window, or a mismatch in the first three bytes), capped at lim =
+// min(258, n - p).  `own4` holds the first four bytes at p.  `hint` carries
+// what the previous position learnt at the same distance (see hint_for):
+// a length known to agree and, in bit 31, whether that length is exact.
+__device__ __forceinline__ uint32_t cand_len(const uint32_t *win, uint32_t WB, uint32_t lim, uint32_t p,
+                                             uint32_t cand, uint32_t window, uint32_t own4, uint32_t hint) {
     if (cand >= p || p - cand > window) return 0;
-    uint32_t lim = n - p;
-    if (lim > MAX_MATCH) lim = MAX_MATCH;
-    uint32_t len = known;
-    if (!known) {
+    uint32_t len = hint & 0x1FFu;
+    if (hint >> 31) return len;
+    if (!len) {
         const uint32_t d0 = own4 ^ ld4(win, cand - WB);
         if (d0 & 0x00FFFFFFu) return 0;
         if (d0) return 3;
@@ -453,14 +452,16 @@ __device__ __forceinline__ uint32_t cand_len(const uint32_t *win, uint32_t WB, u
     return len > lim ? lim : len;
 }
 
-// Known agreement of (p, p - d) from results at p - 1 (len << 16 | dist): a
-// match of length m at p - 1 and distance d means m - 1 equal bytes at p.
-__device__ __forceinline__ uint32_t known_from(uint32_t d, uint32_t h0, uint32_t h1, uint32_t h2) {
-    uint32_t m = 0;
-    if ((h0 & 0xFFFFu) == d) m = h0 >> 16;
-    if ((h1 & 0xFFFFu) == d) m = max(m, h1 >> 16);
-    if ((h2 & 0xFFFFu) == d) m = max(m, h2 >> 16);
-    return m >= 4 ? m - 1 : 0;
+// Result of one candidate at position p - 1 for the next position: length m
+// << 16 | distance, bit 31 set when the match ended at a mismatch (m below
+// that position's cap).  Bytes p-1 .. p-2+m agree with p-1-d .., so at p the
+// same distance agrees over m - 1 bytes, exactly m - 1 when bit 31 is set.
+__device__ __forceinline__ uint32_t hint_rec(uint32_t len, uint32_t dist, uint32_t lim) {
+    return len ? (len << 16) | dist | (len < lim ? 0x80000000u : 0u) : 0u;
+}
+__device__ __forceinline__ uint32_t hint_for(uint32_t d, uint32_t h) {
+    const uint32_t m = (h >> 16) & 0x1FFu;
+    return ((h & 0xFFFFu) == d && m >= 4) ? (m - 1) | (h & 0x80000000u) : 0u;
 }
 
 // The distinct hashes of the query segment get dense ids by rank: a 65536-bit
@@ -478,7 +479,6 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
     __shared__ uint32_t tmp[32];
     const uint32_t tid = threadIdx.x;
     const uint32_t G = uni(w.counts->nseg);
-    const uint2 *tabs = reinterpret_cast<const uint2 *>(w.segtab);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
         const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
         const uint32_t n = md.n, s = g - md.seg_base;
@@ -511,29 +511,13 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
                 reinterpret_cast<uint4 *>(win)[v] = q;
             }
         }
-        for (uint32_t i = tid; i < 2048; i += K1_T) bits[i] = 0;
-        __syncthreads();
-
-        // ---- presence map of the segment's own hashes, then ranks
         {
-            const uint32_t D = uni(w.seg_ntab[g]);
-            const uint2 *T = tabs + (uint64_t)g * SEG;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const uint32_t i = tid + r * K1_T;
-                if (i < D) {
-                    const uint32_t h = T[i].x & 0xFFFFu;
-                    atomicOr(&bits[h >> 5], 1u << (h & 31));
-                }
-            }
-        }
-        __syncthreads();
-        {
-            const uint32_t b0 = bits[2 * tid], b1 = bits[2 * tid + 1];
-            uint32_t tot;
-            const uint32_t ex = block_scan_dpp<K1_T>(__popc(b0) + __popc(b1), tmp, &tot);
-            rnk[2 * tid] = (uint16_t)ex;
-            rnk[2 * tid + 1] = (uint16_t)(ex + __popc(b0));
+            // the segment's own hash bitmap and ranks (2 words / 2 ranks per thread)
+            const uint32_t *T = w.segtab + (uint64_t)g * TAB_WORDS;
+            const uint2 bw = reinterpret_cast<const uint2 *>(T + TAB_BITS)[tid];
+            bits[2 * tid] = bw.x;
+            bits[2 * tid + 1] = bw.y;
+            reinterpret_cast<uint32_t *>(rnk)[tid] = T[TAB_RANK + tid];
         }
         __syncthreads();
 
@@ -558,20 +542,46 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
 
         uint32_t st[4] = {0, 0, 0, 0};  // best len << 16 | dist per position
         for (uint32_t k = 0; k <= kmax; k++) {
-            // stream segment s-k's compact table: every entry whose hash occurs
-            // in this segment stores its (first, last) under the hash's id
+            // segment s-k's table: every hash that also occurs in this segment
+            // stores its (first, last) under its id here, tagged with k
             {
-                const uint32_t gk = g - k;
-                const uint32_t D = uni(w.seg_ntab[gk]);
-                const uint2 *T = tabs + (uint64_t)gk * SEG;
-                const uint32_t tag = k << 24;
+                const uint32_t *T = w.segtab + (uint64_t)(g - k) * TAB_WORDS;
+                if (k == 0) {
+                    // own hashes: ids are ranks, the entries are already in id order
+                    const uint32_t D = uni(w.seg_ntab[g]);
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const uint32_t i = tid + r * K1_T;
-                    if (i >= D) continue;
-                    const uint2 e = T[i];
-                    const uint32_t h = e.x & 0xFFFFu;
-                    if (has_hash(bits, h)) fl[hash_rank(bits, rnk, h)] = tag | ((e.x >> 16) << 12) | e.y;
+                    for (int r = 0; r < 4; r++) {
+                        const uint32_t i = tid + r * K1_T;
+                        if (i < D) fl[i] = T[TAB_ENT + i];
+                    }
+                } else {
+                    const uint2 bk = reinterpret_cast<const uint2 *>(T + TAB_BITS)[tid];
+                    const uint32_t rkk = T[TAB_RANK + tid];
+                    const uint32_t o0 = bits[2 * tid], o1 = bits[2 * tid + 1];
+                    uint32_t I0 = o0 & bk.x, I1 = o1 & bk.y;
+                    if (I0 | I1) {
+                        const uint32_t ro0 = rnk[2 * tid], ro1 = rnk[2 * tid + 1];
+                        const uint32_t tag = k << 24;
+                        // one common hash of each word per step (two independent loads)
+                        while (I0 | I1) {
+                            uint32_t i0 = 0, i1 = 0, q0 = EMPTY, q1 = EMPTY;
+                            if (I0) {
+                                const uint32_t m = (1u << __builtin_ctz(I0)) - 1u;
+                                q0 = ro0 + __popc(o0 & m);
+                                i0 = (rkk & 0xFFFFu) + __popc(bk.x & m);
+                                I0 &= I0 - 1u;
+                            }
+                            if (I1) {
+                                const uint32_t m = (1u << __builtin_ctz(I1)) - 1u;
+                                q1 = ro1 + __popc(o1 & m);
+                                i1 = (rkk >> 16) + __popc(bk.y & m);
+                                I1 &= I1 - 1u;
+                            }
+                            const uint32_t e0 = T[TAB_ENT + i0], e1 = T[TAB_ENT + i1];
+                            if (q0 != EMPTY) fl[q0] = tag | e0;
+                            if (q1 != EMPTY) fl[q1] = tag | e1;
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -584,7 +594,7 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
 #pragma unroll
             for (int i = 0; i < 4; i++)
                 fv[i] = (((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH) ? fl[qid[i]] : EMPTY;
-            uint32_t hb = 0, hn = 0, ho = 0;  // previous position: best, newest, oldest (len << 16 | dist)
+            uint32_t hn = 0, ho = 0;  // previous position's newest / oldest candidate (hint_rec)
 #pragma unroll 1
             for (uint32_t i = 0; i < 4; i++) {
                 const uint32_t v = fv[0];
@@ -592,23 +602,22 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
                 uint32_t rn = 0, ro = 0;
                 if ((v >> 24) == k) {
                     const uint32_t p = S + x0 + i;
+                    const uint32_t lim = min(n - p, MAX_MATCH);
                     const uint32_t own4 = __builtin_amdgcn_alignbyte(own_hi, own_lo, i);
                     const uint32_t cn = base + (v & 0xFFFu), co = base + ((v >> 12) & 0xFFFu);
                     const uint32_t dn = p - cn, dd = p - co;
-                    const uint32_t ln = cand_len(win, WB, n, p, cn, w.window, own4, known_from(dn, hb, hn, ho));
-                    rn = (ln << 16) | (dn & 0xFFFFu);
+                    const uint32_t ln = cand_len(win, WB, lim, p, cn, w.window, own4, hint_for(dn, hn));
+                    rn = hint_rec(ln, dn, lim);
                     if (ln > (cs >> 16)) cs = (ln << 16) | dn;
                     // the oldest entry is tried only while best < 16
                     // (lz77_hash_lookup.wgsl:102); equal to the newest it cannot
                     // give a strictly longer match
                     if ((cs >> 16) < OLDEST_SKIP && co != cn) {
-                        const uint32_t lo =
-                            cand_len(win, WB, n, p, co, w.window, own4, known_from(dd, hb, hn, ho));
-                        ro = (lo << 16) | (dd & 0xFFFFu);
+                        const uint32_t lo = cand_len(win, WB, lim, p, co, w.window, own4, hint_for(dd, ho));
+                        ro = hint_rec(lo, dd, lim);
                         if (lo > (cs >> 16)) cs = (lo << 16) | dd;
                     }
                 }
-                hb = cs;
                 hn = rn;
                 ho = ro;
                 fv[0] = fv[1]; fv[1] = fv[2]; fv[2] = fv[3];

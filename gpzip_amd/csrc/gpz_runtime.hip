@@ -172,7 +172,7 @@ int Workspace::reserve(const Caps &want) {
     L.counts = carve(cur, sizeof(Counts));
     L.desc_end = cur;
     L.tok = carve(cur, c.P * 4);
-    L.segtab = carve(cur, (uint64_t)c.G * SEG * 8);
+    L.segtab = carve(cur, (uint64_t)c.G * TAB_WORDS * 4);
     L.seg_ntab = carve(cur, c.G * 4);
     L.wsum = carve(cur, (uint64_t)c.G * SPAN * 4);
     L.seg_crc = carve(cur, c.G * 4);
@@ -207,7 +207,7 @@ int Workspace::reserve(const Caps &want) {
     w.chunk_mem0 = (const uint32_t *)(b + L.chunk_mem0);
     w.counts = (const Counts *)(b + L.counts);
     w.tok = (uint32_t *)(b + L.tok);
-    w.segtab = (uint64_t *)(b + L.segtab);
+    w.segtab = (uint32_t *)(b + L.segtab);
     w.seg_ntab = (uint32_t *)(b + L.seg_ntab);
     w.wsum = (uint32_t *)(b + L.wsum);
     w.seg_crc = (uint32_t *)(b + L.seg_crc);
