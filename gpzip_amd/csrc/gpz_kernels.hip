@@ -415,40 +415,41 @@ constexpr int WIN_WORDS = 9344;  // 37,376 bytes: 32 KiB back + 4 KiB segment + 
 
 // Length of the match at cand (0 if the reference rejects it: cand >= p,
 // dist > window, or a mismatch in the first three bytes), capped at lim =
-// min(258, n - p).  `own4` holds the first four bytes at p.  `hint` carries
-// what the previous position learnt at the same distance (see hint_for):
-// a length known to agree and, in bit 31, whether that length is exact.
+// min(258, n - p).  own0 / own1 hold bytes p .. p+3 / p+4 .. p+7.  `hint`
+// carries what the previous position learnt at the same distance (see
+// hint_for): a length known to agree and, in bit 31, whether it is exact.
 __device__ __forceinline__ uint32_t cand_len(const uint32_t *win, uint32_t WB, uint32_t lim, uint32_t p,
-                                             uint32_t cand, uint32_t window, uint32_t own4, uint32_t hint) {
+                                             uint32_t cand, uint32_t window, uint32_t own0, uint32_t own1,
+                                             uint32_t hint) {
     if (cand >= p || p - cand > window) return 0;
     uint32_t len = hint & 0x1FFu;
     if (hint >> 31) return len;
     if (!len) {
-        const uint32_t d0 = own4 ^ ld4(win, cand - WB);
-        if (d0 & 0x00FFFFFFu) return 0;
-        if (d0) return 3;
-        len = 4;
+        // first eight bytes in one step: the three-byte check and the short matches
+        const uint32_t c = cand - WB, ic = c >> 2, sc = c & 3u;
+        const uint32_t w0 = win[ic], w1 = win[ic + 1], w2 = win[ic + 2];
+        const uint32_t x0 = own0 ^ __builtin_amdgcn_alignbyte(w1, w0, sc);
+        if (x0 & 0x00FFFFFFu) return 0;
+        if (x0) return 3;
+        const uint32_t x1 = own1 ^ __builtin_amdgcn_alignbyte(w2, w1, sc);
+        if (x1) return min(4 + ((uint32_t)__builtin_ctz(x1) >> 3), lim);
+        if (lim <= 8) return lim;
+        len = 8;
     }
-    // 8 bytes per step: dwords of both sides realigned with alignbyte; one
-    // exit (mismatch found or the cap reached)
+    // 8 bytes per step: three dwords of each side realigned with alignbyte
+    // (re-read rather than carried, so the loop keeps no copies); one exit
     const uint32_t ap = p - WB + len, ac = cand - WB + len;
     const uint32_t sa = ap & 3u, sc = ac & 3u;
-    uint32_t ia = ap >> 2, ic = ac >> 2;
-    uint32_t a0 = win[ia], c0 = win[ic];
-    for (;;) {
-        const uint32_t a1 = win[ia + 1], a2 = win[ia + 2], c1 = win[ic + 1], c2 = win[ic + 2];
-        const uint32_t x0 = __builtin_amdgcn_alignbyte(a1, a0, sa) ^ __builtin_amdgcn_alignbyte(c1, c0, sc);
-        const uint32_t x1 = __builtin_amdgcn_alignbyte(a2, a1, sa) ^ __builtin_amdgcn_alignbyte(c2, c1, sc);
-        if ((x0 | x1) != 0 || len + 8 >= lim) {
-            len += x0 ? (uint32_t)__builtin_ctz(x0) >> 3 : (x1 ? 4 + ((uint32_t)__builtin_ctz(x1) >> 3) : 8);
-            break;
-        }
-        len += 8;
-        ia += 2;
-        ic += 2;
-        a0 = a2;
-        c0 = c2;
-    }
+    const uint32_t *pa = win + (ap >> 2), *pc = win + (ac >> 2);
+    uint32_t j = 0, x0, x1;
+    do {
+        x0 = __builtin_amdgcn_alignbyte(pa[j + 1], pa[j], sa) ^ __builtin_amdgcn_alignbyte(pc[j + 1], pc[j], sc);
+        x1 = __builtin_amdgcn_alignbyte(pa[j + 2], pa[j + 1], sa) ^
+             __builtin_amdgcn_alignbyte(pc[j + 2], pc[j + 1], sc);
+        j += 2;
+    } while ((x0 | x1) == 0 && len + 4 * j < lim);
+    len += 4 * j - 8;
+    len += x0 ? (uint32_t)__builtin_ctz(x0) >> 3 : (x1 ? 4 + ((uint32_t)__builtin_ctz(x1) >> 3) : 8);
     return len > lim ? lim : len;
 }
 
@@ -527,6 +528,7 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
         uint32_t qid[4];
         uint32_t hvm = 0;  // bit i: position x0+i can start a match (p + 2 < n)
         const uint32_t own_lo = win[lx >> 2], own_hi = win[(lx >> 2) + 1];  // bytes x0 .. x0+7
+        const uint32_t own_w2 = win[(lx >> 2) + 2];                          // bytes x0+8 .. x0+11
         {
             const uint64_t vv = (uint64_t)own_lo | ((uint64_t)own_hi << 32);
 #pragma unroll
@@ -603,17 +605,18 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
                 if ((v >> 24) == k) {
                     const uint32_t p = S + x0 + i;
                     const uint32_t lim = min(n - p, MAX_MATCH);
-                    const uint32_t own4 = __builtin_amdgcn_alignbyte(own_hi, own_lo, i);
+                    const uint32_t own0 = __builtin_amdgcn_alignbyte(own_hi, own_lo, i);
+                    const uint32_t own1 = __builtin_amdgcn_alignbyte(own_w2, own_hi, i);
                     const uint32_t cn = base + (v & 0xFFFu), co = base + ((v >> 12) & 0xFFFu);
                     const uint32_t dn = p - cn, dd = p - co;
-                    const uint32_t ln = cand_len(win, WB, lim, p, cn, w.window, own4, hint_for(dn, hn));
+                    const uint32_t ln = cand_len(win, WB, lim, p, cn, w.window, own0, own1, hint_for(dn, hn));
                     rn = hint_rec(ln, dn, lim);
                     if (ln > (cs >> 16)) cs = (ln << 16) | dn;
                     // the oldest entry is tried only while best < 16
                     // (lz77_hash_lookup.wgsl:102); equal to the newest it cannot
                     // give a strictly longer match
                     if ((cs >> 16) < OLDEST_SKIP && co != cn) {
-                        const uint32_t lo = cand_len(win, WB, lim, p, co, w.window, own4, hint_for(dd, ho));
+                        const uint32_t lo = cand_len(win, WB, lim, p, co, w.window, own0, own1, hint_for(dd, ho));
                         ro = hint_rec(lo, dd, lim);
                         if (lo > (cs >> 16)) cs = (lo << 16) | dd;
                     }
