@@ -660,7 +660,7 @@ constexpr int KW_T = 1024;
 
 __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
     __shared__ uint32_t tokL[SEG];
-    __shared__ uint16_t Ea[SEG], Ca[SEG];
+    __shared__ uint16_t Ea[2][SEG], Ca[2][SEG];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t G = uni(w.counts->nseg);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
@@ -685,7 +685,8 @@ __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
             __syncthreads();
             continue;
         }
-        // level 1: 16 waves x 4 blocks
+        // level 1: 16 waves x 4 blocks; per position the exit from its
+        // 64-position block and the tokens emitted inside it
         for (uint32_t b = wid; b < 64; b += KW_T / 64) {
             const uint32_t bs = b * 64;
             if (bs >= L) break;
@@ -696,20 +697,30 @@ __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
             uint64_t M;
             const uint32_t ex = block_walk(J, bs, L, valid, valid && x == absorb, M);
             if (valid) {
-                Ea[x] = (uint16_t)ex;
-                Ca[x] = (uint16_t)__popcll(M);
+                Ea[0][x] = (uint16_t)ex;
+                Ca[0][x] = (uint16_t)__popcll(M);
             }
         }
         __syncthreads();
-        // level 2: each entry follows the block exits (at most 64 hops)
-        for (uint32_t x = tid; x < L && x < SPAN; x += KW_T) {
-            uint32_t y = x, c = 0;
-            while (y < L && y != absorb) {
-                c += Ca[y];
-                y = Ea[y];
+        // level 2: pointer doubling over the block exits.  Every exit lies in
+        // a later block, so 64 hops (6 rounds) reach the segment end or the
+        // absorbing last position (its own exit, count 0).
+        uint32_t cur = 0;
+        for (int r = 0; r < 6; r++) {
+            for (uint32_t x = tid; x < L; x += KW_T) {
+                const uint32_t y = Ea[cur][x];
+                uint32_t e2 = y, c2 = Ca[cur][x];
+                if (y < L) {
+                    e2 = Ea[cur][y];
+                    c2 += Ca[cur][y];
+                }
+                Ea[cur ^ 1][x] = (uint16_t)e2;
+                Ca[cur ^ 1][x] = (uint16_t)c2;
             }
-            out[x] = y | (c << 16);
+            __syncthreads();
+            cur ^= 1;
         }
+        for (uint32_t x = tid; x < L && x < SPAN; x += KW_T) out[x] = Ea[cur][x] | ((uint32_t)Ca[cur][x] << 16);
         __syncthreads();
     }
 }
@@ -789,9 +800,10 @@ constexpr int K3_T = 1024;
 
 __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
     __shared__ uint32_t tokL[SEG + 4];
-    __shared__ uint16_t Ja[SEG];
-    __shared__ uint8_t mark[SEG];
-    __shared__ uint64_t vb[64];
+    __shared__ uint64_t Mv[SEG];     // per position: positions walked inside its block from it
+    __shared__ uint16_t Ja[SEG];     // per position: exit from its block
+    __shared__ uint64_t vb[64];      // walked positions of each block
+    __shared__ uint64_t demb[64];    // demoted matches of each block (lazy rule)
     __shared__ uint32_t cntb[64];
     __shared__ uint32_t hist[NSYM];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -802,74 +814,48 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
         const MemberDesc md = load_member(w.mem, m);
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
-        const uint32_t e = w.seg_entry[g];
+        const uint32_t e = uni(w.seg_entry[g]);
         if (e >= L) continue;  // the walk jumps over this segment (uniform)
         const bool next_known = E < n;
         const uint32_t tokbase = w.seg_tokbase[g];
         const uint32_t *tg = w.tok + md.pos_base + S;
-        for (uint32_t x = tid; x < L; x += K3_T) {
-            tokL[x] = tg[x];
-            mark[x] = 0;
-        }
+        for (uint32_t x = tid; x < L; x += K3_T) tokL[x] = tg[x];
         if (tid == 0) tokL[L] = next_known ? tg[L] : 0;
         for (uint32_t i = tid; i < NSYM; i += K3_T) hist[i] = 0;
+        if (tid < 64) vb[tid] = 0;
         __syncthreads();
-        // level 1: block exits
+        // level 1: per position its block exit and the positions walked
+        // inside the block from it
         for (uint32_t b = wid; b < 64; b += K3_T / 64) {
             const uint32_t bs = b * 64;
             if (bs >= L) break;
             const uint32_t x = bs + lane;
             const bool valid = x < L;
-            bool dem;
+            bool dem = false;
             const uint32_t J = valid ? succ_of(tokL, x, L, next_known, w.strict, false, dem) : 0;
             uint64_t M;
             const uint32_t ex = block_walk(J, bs, L, valid, false, M);
-            if (valid) Ja[x] = (uint16_t)ex;
+            if (valid) {
+                Ja[x] = (uint16_t)ex;
+                Mv[x] = M;
+            }
+            const uint64_t dm = __ballot(valid && dem);
+            if (lane == 0) demb[b] = dm;
         }
         __syncthreads();
-        // level 2: one lane follows the block exits from the entry (at most 64 hops)
+        // level 2: one lane follows the block exits from the entry (at most
+        // 64 hops); the entry of each block it meets selects the block's set
         if (tid == 0) {
             uint32_t y = e;
             while (y < L) {
-                mark[y] = 1;
+                vb[y >> 6] = Mv[y];
                 y = Ja[y];
             }
         }
         __syncthreads();
-        // visited set of each block from its marked entry
-        for (uint32_t b = wid; b < 64; b += K3_T / 64) {
-            const uint32_t bs = b * 64;
-            uint64_t V = 0;
-            if (bs < L) {
-                const uint32_t x = bs + lane;
-                const bool valid = x < L;
-                const uint64_t em = __ballot(valid && mark[x]);
-                if (em) {
-                    bool dem;
-                    const uint32_t J = valid ? succ_of(tokL, x, L, next_known, w.strict, false, dem) : 0;
-                    uint64_t M;
-                    block_walk(J, bs, L, valid, false, M);
-                    const int el = __builtin_ctzll(em);
-                    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)M, el, 64);
-                    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(M >> 32), el, 64);
-                    V = ((uint64_t)hi << 32) | lo;
-                }
-            }
-            if (lane == 0) {
-                vb[b] = V;
-                cntb[b] = __popcll(V);
-            }
-        }
-        __syncthreads();
         if (wid == 0) {  // exclusive scan of the 64 block counts
-            const uint32_t c = cntb[lane];
-            uint32_t x = c;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t y = __shfl_up(x, d, 64);
-                if (lane >= (uint32_t)d) x += y;
-            }
-            cntb[lane] = x - c;
+            const uint32_t c = __popcll(vb[lane]);
+            cntb[lane] = wave_scan_incl(c) - c;
         }
         __syncthreads();
         uint32_t *outp = w.walked + md.pos_base + tokbase;
@@ -879,10 +865,8 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
             const uint64_t V = vb[b];
             if (!((V >> lane) & 1)) continue;
             const uint32_t x = bs + lane;
-            bool dem;
-            succ_of(tokL, x, L, next_known, w.strict, false, dem);
             uint32_t t = tokL[x];
-            if (dem) t = w.in[md.in_off + S + x];
+            if ((demb[b] >> lane) & 1) t = w.in[md.in_off + S + x];
             outp[cntb[b] + __popcll(V & lt)] = t;
             const uint32_t len = t >> 16, v = t & 0xFFFFu;
             if (len == 0) {
