@@ -185,17 +185,24 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_re
 __device__ __forceinline__ uint64_t uni64(uint64_t v) {
     return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v);
 }
-// Member descriptor m, read by a whole workgroup (m must be uniform).
+// Member descriptor m, read by a whole workgroup (m must be uniform), with
+// one scalar load into scalar registers.  The compiler cannot prove the
+// descriptor array unwritten inside kernels that store to global memory, so
+// it would otherwise load it into vector registers (and spill them).  The
+// wait is inside the asm block: the outputs are complete when it ends.
+typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ MemberDesc load_member(const MemberDesc *mem, uint32_t m) {
-    const MemberDesc d = mem[m];
-    MemberDesc r;
-    r.in_off = uni64(d.in_off);
-    r.pos_base = uni64(d.pos_base);
-    r.n = uni(d.n);
-    r.seg_base = uni(d.seg_base);
-    r.nseg = uni(d.nseg);
-    r.chunk = uni(d.chunk);
-    return r;
+    const MemberDesc *p = mem + m;
+    u32x8 r;
+    asm volatile("s_load_dwordx8 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=s"(r) : "s"(p) : "memory");
+    MemberDesc d;
+    d.in_off = (uint64_t)r[0] | ((uint64_t)r[1] << 32);
+    d.pos_base = (uint64_t)r[2] | ((uint64_t)r[3] << 32);
+    d.n = r[4];
+    d.seg_base = r[5];
+    d.nseg = r[6];
+    d.chunk = r[7];
+    return d;
 }
 
 // Spread consecutive work items of a persistent grid so that each XCD (blocks
@@ -635,8 +642,9 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
         // ---- tokens
         uint32_t t4[4];
         {
+            const uint32_t ob = win[lx >> 2];  // re-read: keeping own_lo alive costs a spill
 #pragma unroll
-            for (int i = 0; i < 4; i++) t4[i] = st[i] ? st[i] : (own_lo >> (8 * i)) & 0xFF;
+            for (int i = 0; i < 4; i++) t4[i] = st[i] ? st[i] : (ob >> (8 * i)) & 0xFF;
         }
         uint32_t *tg = w.tok + md.pos_base + S + x0;
         if (x0 + 4 <= L) {
