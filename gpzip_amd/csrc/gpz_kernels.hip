@@ -919,90 +919,133 @@ __global__ __launch_bounds__(1024) void k_hist_tokens(Work w) {
 // k_huffman: one wave per member
 
 struct HuffScratch {
-    uint32_t work[288];
-    uint32_t sf[288];       // leaf frequencies in (freq, id) order
-    uint32_t ifreq[288];    // internal node frequencies, creation order
-    uint16_t sorted[288];
+    uint32_t key[288];      // leaf (freq << 9 | id), 0xFFFFFFFF when unused
+    uint32_t sk[320];       // keys in sorted order
     uint16_t par[2][576];   // node parent; leaf i -> node i, internal k -> node 288 + k
     uint16_t dep[2][576];
-    uint32_t misc[4];
 };
 
-// try_build_code_lengths (huffman.rs:26-60).  The reference pops the minimum
-// (freq, id) from a binary heap whose internal nodes get ids n, n+1, ...;
-// merged frequencies never decrease and internal ids exceed every leaf id,
-// so merging a (freq, id)-sorted leaf list with a FIFO of internal nodes
-// (leaf first on equal frequency) pops the same sequence.  The merge runs on
-// lane 0 from registers; depths (= code lengths) come from pointer jumping
-// over the parent links on all 64 lanes.
-__device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens, HuffScratch &hs) {
-    const int lane = threadIdx.x;
-    uint32_t ucnt = 0;
-    for (int i = lane; i < n; i += 64) {
-        hs.work[i] = f[i];
-        ucnt += f[i] > 0;
+// Wave-uniform element t of a 5-register array spread over the lanes
+// (element t in lane t % 64 of register t / 64).
+__device__ __forceinline__ uint32_t rd5(const uint32_t (&a)[5], uint32_t t) {
+    const uint32_t l = t & 63u;
+    switch (t >> 6) {
+        case 0: return __builtin_amdgcn_readlane(a[0], l);
+        case 1: return __builtin_amdgcn_readlane(a[1], l);
+        case 2: return __builtin_amdgcn_readlane(a[2], l);
+        case 3: return __builtin_amdgcn_readlane(a[3], l);
+        default: return __builtin_amdgcn_readlane(a[4], l);
     }
-    for (int d = 32; d >= 1; d >>= 1) ucnt += __shfl_xor(ucnt, d, 64);
-    const uint32_t used = ucnt;
-    __syncthreads();
+}
+__device__ __forceinline__ void wr5(uint32_t (&a)[5], uint32_t t, uint32_t v) {
+    const bool mine = (threadIdx.x & 63u) == (t & 63u);
+    switch (t >> 6) {
+        case 0: a[0] = mine ? v : a[0]; break;
+        case 1: a[1] = mine ? v : a[1]; break;
+        case 2: a[2] = mine ? v : a[2]; break;
+        case 3: a[3] = mine ? v : a[3]; break;
+        default: a[4] = mine ? v : a[4]; break;
+    }
+}
+
+// try_build_code_lengths (huffman.rs:26-60) for n <= 288 symbols on one
+// wave.  The reference pops the minimum (freq, id) from a binary heap whose
+// internal nodes get ids n, n+1, ...; merged frequencies never decrease and
+// internal ids exceed every leaf id, so merging the (freq, id)-sorted leaves
+// with a FIFO of internal nodes (leaf first on equal frequency) pops the same
+// sequence.  Leaves are ranked by all lanes; the merge state is wave-uniform
+// (scalar registers) with the sorted leaves and the internal-node FIFO held
+// one element per lane in registers; depths (= code lengths) come from
+// pointer jumping over the parent links.  Frequencies are halved while the
+// longest code exceeds `limit` (huffman.rs:44-58).
+__device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens, HuffScratch &hs) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t fr[5];
+    uint32_t used = 0;
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        const uint32_t i = lane + 64 * j;
+        fr[j] = (int)i < n ? f[i] : 0u;
+        used += __popcll(__ballot(fr[j] != 0));
+    }
+    used = uni(used);
     if (used <= 1) {
-        for (int i = lane; i < n; i += 64) lens[i] = f[i] > 0 ? 1 : 0;
+#pragma unroll
+        for (int j = 0; j < 5; j++)
+            if ((int)(lane + 64 * j) < n) lens[lane + 64 * j] = fr[j] ? 1 : 0;
         __syncthreads();
         return;
     }
     const uint32_t root = 288 + used - 2;
     for (;;) {
-        for (int i = lane; i < n; i += 64) {
-            const uint32_t wi = hs.work[i];
-            if (!wi) continue;
-            uint32_t r = 0;
-            for (int j = 0; j < n; j++) {
-                const uint32_t wj = hs.work[j];
-                r += (wj != 0) & ((wj < wi) | ((wj == wi) & (j < i)));
-            }
-            hs.sorted[r] = (uint16_t)i;
-            hs.sf[r] = wi;
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            const uint32_t i = lane + 64 * j;
+            if ((int)i < n) hs.key[i] = fr[j] ? (fr[j] << 9) | i : 0xFFFFFFFFu;
         }
         __syncthreads();
-        if (lane == 0) {
+        // rank of each used leaf among the used leaves by (freq, id)
+        {
+            uint32_t kk[5], r[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                kk[j] = (int)(lane + 64 * j) < n ? hs.key[lane + 64 * j] : 0xFFFFFFFFu;
+                r[j] = 0;
+            }
+            for (int i = 0; i < n; i++) {
+                const uint32_t ki = hs.key[i];
+#pragma unroll
+                for (int j = 0; j < 5; j++) r[j] += ki < kk[j];
+            }
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                if (kk[j] != 0xFFFFFFFFu) hs.sk[r[j]] = kk[j];
+        }
+        __syncthreads();
+        uint32_t sk[5], iq[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            const uint32_t t = lane + 64 * j;
+            sk[j] = t < used ? hs.sk[t] : 0xFFFFFFFFu;
+            iq[j] = 0;
+        }
+        // merge: wave-uniform control, one pick per step from the leaf list
+        // or the internal FIFO
+        {
             uint32_t nl = 0, qh = 0;
-            uint32_t lf = hs.sf[0];
-            uint32_t lid = hs.sorted[0];
+            uint32_t lkey = rd5(sk, 0);
             for (uint32_t k = 0; k + 1 < used; k++) {
                 uint32_t fsum = 0;
 #pragma unroll
                 for (int pick = 0; pick < 2; pick++) {
                     const bool have_int = qh < k;
-                    const uint32_t qf = have_int ? hs.ifreq[qh] : 0;
-                    if (nl < used && (!have_int || lf <= qf)) {
-                        fsum += lf;
-                        hs.par[0][lid] = (uint16_t)(288 + k);
+                    const uint32_t qf = have_int ? rd5(iq, qh) : 0u;
+                    if (nl < used && (!have_int || (lkey >> 9) <= qf)) {
+                        fsum += lkey >> 9;
+                        if (lane == 0) hs.par[0][lkey & 0x1FFu] = (uint16_t)(288 + k);
                         nl++;
-                        if (nl < used) {
-                            lf = hs.sf[nl];
-                            lid = hs.sorted[nl];
-                        }
+                        lkey = nl < used ? rd5(sk, nl) : 0xFFFFFFFFu;
                     } else {
                         fsum += qf;
-                        hs.par[0][288 + qh] = (uint16_t)(288 + k);
+                        if (lane == 0) hs.par[0][288 + qh] = (uint16_t)(288 + k);
                         qh++;
                     }
                 }
-                hs.ifreq[k] = fsum;
+                wr5(iq, k, fsum);
             }
-            hs.par[0][root] = (uint16_t)root;
+            if (lane == 0) hs.par[0][root] = (uint16_t)root;
         }
         __syncthreads();
         // depth of every node: pointer jumping, ceil(log2(286)) = 9 rounds
-        for (int id = lane; id < 576; id += 64) {
-            const bool valid = id < 288 ? (id < n && hs.work[id] != 0) : (uint32_t)(id - 288) + 1 < used;
-            if (valid) hs.dep[0][id] = id == (int)root ? 0 : 1;
+        for (uint32_t id = lane; id < 576; id += 64) {
+            const bool valid = id < 288 ? ((int)id < n && hs.key[id] != 0xFFFFFFFFu) : (id - 288) + 1 < used;
+            if (valid) hs.dep[0][id] = id == root ? 0 : 1;
         }
         __syncthreads();
         int cur = 0;
         for (int r = 0; r < 9; r++) {
-            for (int id = lane; id < 576; id += 64) {
-                const bool valid = id < 288 ? (id < n && hs.work[id] != 0) : (uint32_t)(id - 288) + 1 < used;
+            for (uint32_t id = lane; id < 576; id += 64) {
+                const bool valid = id < 288 ? ((int)id < n && hs.key[id] != 0xFFFFFFFFu) : (id - 288) + 1 < used;
                 if (!valid) continue;
                 const uint32_t pa = hs.par[cur][id];
                 hs.dep[cur ^ 1][id] = hs.dep[cur][id] + hs.dep[cur][pa];
@@ -1012,18 +1055,21 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
             cur ^= 1;
         }
         uint32_t mx = 0;
-        for (int i = lane; i < n; i += 64)
-            if (hs.work[i]) mx = max(mx, (uint32_t)hs.dep[cur][i]);
+#pragma unroll
+        for (int j = 0; j < 5; j++)
+            if (fr[j]) mx = max(mx, (uint32_t)hs.dep[cur][lane + 64 * j]);
         for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
         if (mx <= (uint32_t)limit) {
-            for (int i = lane; i < n; i += 64) lens[i] = hs.work[i] ? (uint8_t)hs.dep[cur][i] : 0;
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                if ((int)(lane + 64 * j) < n) lens[lane + 64 * j] = fr[j] ? (uint8_t)hs.dep[cur][lane + 64 * j] : 0;
             __syncthreads();
             return;
         }
         __syncthreads();
-        for (int i = lane; i < n; i += 64)
-            if (hs.work[i] > 1) hs.work[i] >>= 1;
-        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 5; j++)
+            if (fr[j] > 1) fr[j] >>= 1;
     }
 }
 
@@ -1063,26 +1109,54 @@ __device__ void canonical_rev(const uint8_t *lens, int n, uint32_t *codes) {
     __syncthreads();
 }
 
-// LSB-first bit writer with a register accumulator (lane 0 only).
-struct BitBuf {
-    uint32_t *words;
-    uint64_t acc = 0;
-    uint32_t nacc = 0, nw = 0;
-    __device__ void put(uint32_t v, uint32_t nb) {
-        acc |= (uint64_t)v << nacc;
-        nacc += nb;
-        if (nacc >= 32) {
-            words[nw++] = (uint32_t)acc;
-            acc >>= 32;
-            nacc -= 32;
+// Codes of one run of equal code lengths (rle_encode_lengths,
+// deflate.rs:254-326): a zero run becomes 18s (11-138 zeros), then one 17
+// (3-10), then single zeros; a run of v becomes v, then 16s (3-6 copies),
+// then single v.  Entries are sym | extra bits << 8 | extra value << 16.
+__device__ __forceinline__ uint32_t rle_count(uint32_t v, uint32_t R) {
+    if (v == 0) {
+        const uint32_t a = R / 138, b = R % 138;
+        const uint32_t n18 = a + (b >= 11), r1 = b >= 11 ? 0 : b;
+        return n18 + (r1 >= 3 ? 1 : r1);
+    }
+    const uint32_t r = R - 1;
+    return 1 + r / 6 + ((r % 6) >= 3 ? 1 : r % 6);
+}
+__device__ __forceinline__ void rle_write(uint32_t v, uint32_t R, uint32_t *out) {
+    uint32_t o = 0;
+    if (v == 0) {
+        const uint32_t a = R / 138, b = R % 138;
+        for (uint32_t i = 0; i < a; i++) out[o++] = 18u | (7u << 8) | (127u << 16);
+        uint32_t r1 = b;
+        if (b >= 11) {
+            out[o++] = 18u | (7u << 8) | ((b - 11) << 16);
+            r1 = 0;
         }
+        if (r1 >= 3) {
+            out[o++] = 17u | (3u << 8) | ((r1 - 3) << 16);
+        } else {
+            for (uint32_t i = 0; i < r1; i++) out[o++] = 0;
+        }
+        return;
     }
-    __device__ uint32_t finish() {
-        const uint32_t bits = nw * 32 + nacc;
-        if (nacc) words[nw++] = (uint32_t)acc;
-        return bits;
+    out[o++] = v;
+    const uint32_t r = R - 1;
+    for (uint32_t i = 0; i < r / 6; i++) out[o++] = 16u | (2u << 8) | (3u << 16);
+    const uint32_t t = r % 6;
+    if (t >= 3) {
+        out[o++] = 16u | (2u << 8) | ((t - 3) << 16);
+    } else {
+        for (uint32_t i = 0; i < t; i++) out[o++] = v;
     }
-};
+}
+
+// OR the low nb bits of v into an LSB-first bit buffer at bit position pos.
+__device__ __forceinline__ void put_bits(uint32_t *buf, uint32_t pos, uint32_t v, uint32_t nb) {
+    if (!nb) return;
+    const uint32_t w = pos >> 5, sh = pos & 31u;
+    atomicOr(&buf[w], v << sh);
+    if (sh + nb > 32) atomicOr(&buf[w + 1], v >> (32 - sh));
+}
 
 __global__ __launch_bounds__(64) void k_huffman(Work w) {
     __shared__ uint32_t freq[NSYM];
@@ -1094,14 +1168,15 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
     __shared__ uint8_t mlens[20];
     __shared__ uint32_t mcodes[20];
     __shared__ uint32_t pbuf[PREFIX_WORDS + 2];
-    __shared__ uint32_t info[8];
     __shared__ HuffScratch hs;
     const uint32_t M = uni(w.counts->nmem);
-    const int lane = threadIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t lt = (1ull << lane) - 1;
     for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
         const MemberDesc md = load_member(w.mem, m);
-        for (int i = lane; i < (int)NSYM; i += 64) freq[i] = w.hist[(uint64_t)m * NSYM + i];
-        for (int i = lane; i < 20; i += 64) mfreq[i] = 0;
+        for (uint32_t i = lane; i < NSYM; i += 64) freq[i] = w.hist[(uint64_t)m * NSYM + i];
+        for (uint32_t i = lane; i < 20; i += 64) mfreq[i] = 0;
+        for (uint32_t i = lane; i < PREFIX_WORDS + 2; i += 64) pbuf[i] = 0;
         __syncthreads();
         if (lane == 0) freq[256] += 1;  // end of block
         __syncthreads();
@@ -1111,7 +1186,7 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
             // single/zero distance-code fix-up and litlen fix-up (deflate.rs:855-867)
             const uint64_t dm = __ballot(lane < 30 && lens[DSYM0 + lane] != 0);
             uint32_t used_ll = 0;
-            for (int i = lane; i < 286; i += 64) used_ll += lens[i] != 0;
+            for (uint32_t i = lane; i < 286; i += 64) used_ll += lens[i] != 0;
             for (int d = 32; d >= 1; d >>= 1) used_ll += __shfl_xor(used_ll, d, 64);
             __syncthreads();
             if (lane == 0) {
@@ -1133,106 +1208,124 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
         }
         canonical_rev(lens, 286, codes);
         canonical_rev(lens + DSYM0, 30, codes + DSYM0);
-        if (lane == 0) {
-            // trim_to_min (deflate.rs:329-335) and RLE (deflate.rs:254-326)
-            uint32_t nlit = 286, ndist = 30;
-            while (nlit > 257 && lens[nlit - 1] == 0) nlit--;
-            while (ndist > 1 && lens[DSYM0 + ndist - 1] == 0) ndist--;
-            for (uint32_t i = 0; i < nlit; i++) comb[i] = lens[i];
-            for (uint32_t i = 0; i < ndist; i++) comb[nlit + i] = lens[DSYM0 + i];
-            const uint32_t nc = nlit + ndist;
-            uint32_t k = 0, i = 0;
-            while (i < nc) {
-                uint32_t cur = comb[i], run = 1;
-                while (i + run < nc && comb[i + run] == cur) run++;
-                if (cur == 0) {
-                    while (run >= 11) {
-                        uint32_t q = min(run, 138u);
-                        rle[k++] = 18u | (7u << 8) | ((q - 11) << 16);
-                        run -= q;
-                        i += q;
-                    }
-                    while (run >= 3) {
-                        uint32_t q = min(run, 10u);
-                        rle[k++] = 17u | (3u << 8) | ((q - 3) << 16);
-                        run -= q;
-                        i += q;
-                    }
-                    while (run > 0) {
-                        rle[k++] = 0;
-                        run--;
-                        i++;
-                    }
-                } else {
-                    rle[k++] = cur;
-                    i++;
-                    run--;
-                    while (run >= 3) {
-                        uint32_t q = min(run, 6u);
-                        rle[k++] = 16u | (2u << 8) | ((q - 3) << 16);
-                        run -= q;
-                        i += q;
-                    }
-                    while (run > 0) {
-                        rle[k++] = cur;
-                        run--;
-                        i++;
-                    }
-                }
+        // trim_to_min (deflate.rs:329-335): HLIT >= 257, HDIST >= 1
+        uint32_t nlit, ndist;
+        {
+            uint32_t hl = 0, hd = 0;
+            for (uint32_t i = lane; i < 286; i += 64)
+                if (lens[i]) hl = max(hl, i + 1);
+            if (lane < 30 && lens[DSYM0 + lane]) hd = lane + 1;
+            for (int d = 32; d >= 1; d >>= 1) {
+                hl = max(hl, (uint32_t)__shfl_xor(hl, d, 64));
+                hd = max(hd, (uint32_t)__shfl_xor(hd, d, 64));
             }
-            for (uint32_t r = 0; r < k; r++) mfreq[rle[r] & 0xFF]++;
-            info[0] = nlit;
-            info[1] = ndist;
-            info[2] = k;
+            nlit = uni(max(hl, 257u));
+            ndist = uni(max(hd, 1u));
         }
+        const uint32_t nc = nlit + ndist;
+        for (uint32_t i = lane; i < nc; i += 64) comb[i] = i < nlit ? lens[i] : lens[DSYM0 + i - nlit];
+        __syncthreads();
+        // RLE of the combined lengths, one run per starting lane: run starts
+        // by ballot, run lengths from the next start, output slots by scan
+        uint32_t K = 0;
+        {
+            uint64_t B[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                const uint32_t p = lane + 64 * j;
+                B[j] = __ballot(p < nc && (p == 0 || comb[p] != comb[p - 1]));
+            }
+            uint32_t carry = 0;
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                const uint32_t p = lane + 64 * j;
+                const bool st = (B[j] >> lane) & 1;
+                uint32_t R = 0, cnt = 0, v = 0;
+                if (st) {
+                    // next start after p
+                    uint32_t nx = nc;
+                    const uint64_t above = lane == 63 ? 0ull : B[j] & ~((2ull << lane) - 1);
+                    if (above) {
+                        nx = 64 * j + __builtin_ctzll(above);
+                    } else {
+#pragma unroll
+                        for (int j2 = j + 1; j2 < 5; j2++)
+                            if (nx == nc && B[j2]) nx = 64 * j2 + __builtin_ctzll(B[j2]);
+                    }
+                    R = nx - p;
+                    v = comb[p];
+                    cnt = rle_count(v, R);
+                }
+                const uint32_t inc = wave_scan_incl(cnt);
+                if (st) rle_write(v, R, rle + carry + inc - cnt);
+                carry += __builtin_amdgcn_readlane(inc, 63);
+            }
+            K = uni(carry);
+        }
+        __syncthreads();
+        for (uint32_t r = lane; r < K; r += 64) atomicAdd(&mfreq[rle[r] & 0xFF], 1u);
         __syncthreads();
         build_lengths(mfreq, 19, 7, mlens, hs);
         canonical_rev(mlens, 19, mcodes);
+        // dynamic block header (deflate.rs:886-930): all fields at known
+        // offsets, the RLE codes placed by a scan of their bit counts
+        uint32_t hclen;
+        {
+            uint32_t h = 0;
+            if (lane < 19 && mlens[c_meta_order[lane]]) h = lane + 1;
+            for (int d = 32; d >= 1; d >>= 1) h = max(h, (uint32_t)__shfl_xor(h, d, 64));
+            hclen = uni(max(h, 4u));
+        }
+        const uint32_t b0 = w.gzip ? 80u : 0u;
         if (lane == 0) {
-            uint32_t hclen = 19;
-            while (hclen > 4 && mlens[c_meta_order[hclen - 1]] == 0) hclen--;
-            BitBuf bb;
-            bb.words = pbuf;
             if (w.gzip) {
                 // gzip header 1f 8b 08 00 00000000 00 ff (deflate.rs:965-972)
-                bb.put(0x00088b1fu, 32);
-                bb.put(0, 32);
-                bb.put(0xff00u, 16);
+                pbuf[0] = 0x00088b1fu;
+                pbuf[1] = 0;
+                pbuf[2] = 0xff00u;
             }
-            bb.put(5u, 3);  // BFINAL=1, BTYPE=10
-            bb.put(info[0] - 257, 5);
-            bb.put(info[1] - 1, 5);
-            bb.put(hclen - 4, 4);
-            for (uint32_t i = 0; i < hclen; i++) bb.put(mlens[c_meta_order[i]], 3);
-            for (uint32_t r = 0; r < info[2]; r++) {
-                uint32_t c = rle[r] & 0xFF, eb = (rle[r] >> 8) & 0xFF, ev = rle[r] >> 16;
-                bb.put(mcodes[c] & 0xFFFF, mlens[c]);
-                bb.put(ev, eb);
+            put_bits(pbuf, b0, 5u | ((nlit - 257) << 3) | ((ndist - 1) << 8) | ((hclen - 4) << 13), 17);
+        }
+        if (lane < hclen) put_bits(pbuf, b0 + 17 + 3 * lane, mlens[c_meta_order[lane]], 3);
+        uint32_t pbits;
+        {
+            uint32_t pos = b0 + 17 + 3 * hclen;
+            for (uint32_t r0 = 0; r0 < K; r0 += 64) {
+                const uint32_t r = r0 + lane;
+                uint32_t nb = 0, val = 0;
+                if (r < K) {
+                    const uint32_t e = rle[r], c = e & 0xFF, eb = (e >> 8) & 0xFF, ev = e >> 16;
+                    const uint32_t cl = mlens[c];
+                    nb = cl + eb;
+                    val = (mcodes[c] & 0xFFFFu) | (ev << cl);
+                }
+                const uint32_t inc = wave_scan_incl(nb);
+                if (r < K) put_bits(pbuf, pos + inc - nb, val, nb);
+                pos += __builtin_amdgcn_readlane(inc, 63);
             }
-            info[3] = bb.finish();
+            pbits = uni(pos);
         }
         __syncthreads();
         // publish codes / prefix, count body bits
         uint64_t body = 0;
-        for (int i = lane; i < (int)NSYM; i += 64) {
+        for (uint32_t i = lane; i < NSYM; i += 64) {
             uint32_t c = 0;
-            if (i < 286 || (i >= (int)DSYM0 && i < (int)DSYM0 + 30)) c = codes[i];
+            if (i < 286 || (i >= DSYM0 && i < DSYM0 + 30)) c = codes[i];
             w.codes[(uint64_t)m * NSYM + i] = c;
-            uint32_t f = freq[i];
+            const uint32_t f = freq[i];
             if (!f) continue;
             uint32_t extra = 0;
             if (i >= 265 && i < 285) extra = len_extra(i - 257);
-            if (i >= (int)DSYM0 && i < (int)DSYM0 + 30) extra = dist_extra(i - DSYM0);
+            if (i >= DSYM0 && i < DSYM0 + 30) extra = dist_extra(i - DSYM0);
             body += (uint64_t)f * ((c >> 16) + extra);
         }
         for (int d = 32; d >= 1; d >>= 1) body += __shfl_xor(body, d, 64);
-        const uint32_t pbits = info[3];
         for (uint32_t i = lane; i < (pbits + 31) / 32; i += 64) w.prefix[(uint64_t)m * PREFIX_WORDS + i] = pbuf[i];
         // member CRC-32 from the segments' raw CRCs
         uint32_t crc = 0;
         if (w.gzip) {
             for (uint32_t s = lane; s < md.nseg; s += 64) {
-                uint64_t end = min((uint64_t)(s + 1) << SEG_LOG2, (uint64_t)md.n);
+                const uint64_t end = min((uint64_t)(s + 1) << SEG_LOG2, (uint64_t)md.n);
                 crc ^= gf2_mul(w.seg_crc[md.seg_base + s], xpow8(md.n - end));
             }
             for (int d = 32; d >= 1; d >>= 1) crc ^= __shfl_xor(crc, d, 64);
@@ -1240,7 +1333,7 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
         }
         if (lane == 0) {
             w.prefix_bits[m] = pbits;
-            uint64_t bits = pbits + body;
+            const uint64_t bits = pbits + body;
             w.mem_bytes[m] = (bits + 7) / 8 + (w.gzip ? 8 : 0);
             w.mem_crc[m] = crc;
         }
