@@ -40,6 +40,8 @@ namespace gpz {
 __constant__ uint32_t c_crc_tab[256];   // reflected CRC-32 byte table
 __constant__ uint32_t c_crc_fold[512];  // x^(8*8*(511-t)) mod P, reflected
 __constant__ uint32_t c_xpow[32];       // x^(8*2^k) mod P, reflected
+__constant__ uint32_t c_segpow[4097];   // x^(8*4096*j) mod P, reflected (segments of 4 KiB)
+__constant__ uint32_t c_bytepow[8];     // x^(8*b) mod P, reflected, b < 8
 
 __constant__ uint16_t c_len_base[29] = {3,  4,  5,  6,  7,  8,  9,  10, 11,  13,  15,  17,  19,  23, 27,
                                         31, 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
@@ -919,7 +921,7 @@ __global__ __launch_bounds__(1024) void k_hist_tokens(Work w) {
 // k_huffman: one wave per member
 
 struct HuffScratch {
-    uint32_t key[288];      // leaf (freq << 9 | id), 0xFFFFFFFF when unused
+    __align__(16) uint32_t key[288];  // leaf (freq << 9 | id), 0xFFFFFFFF when unused
     uint32_t sk[320];       // keys in sorted order
     uint16_t par[2][576];   // node parent; leaf i -> node i, internal k -> node 288 + k
     uint16_t dep[2][576];
@@ -982,6 +984,7 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
         for (int j = 0; j < 5; j++) {
             const uint32_t i = lane + 64 * j;
             if ((int)i < n) hs.key[i] = fr[j] ? (fr[j] << 9) | i : 0xFFFFFFFFu;
+            else if (i < 288) hs.key[i] = 0xFFFFFFFFu;
         }
         __syncthreads();
         // rank of each used leaf among the used leaves by (freq, id)
@@ -992,10 +995,10 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
                 kk[j] = (int)(lane + 64 * j) < n ? hs.key[lane + 64 * j] : 0xFFFFFFFFu;
                 r[j] = 0;
             }
-            for (int i = 0; i < n; i++) {
-                const uint32_t ki = hs.key[i];
+            for (int i = 0; i < n; i += 4) {
+                const uint4 k4 = *reinterpret_cast<const uint4 *>(&hs.key[i]);
 #pragma unroll
-                for (int j = 0; j < 5; j++) r[j] += ki < kk[j];
+                for (int j = 0; j < 5; j++) r[j] += (k4.x < kk[j]) + (k4.y < kk[j]) + (k4.z < kk[j]) + (k4.w < kk[j]);
             }
 #pragma unroll
             for (int j = 0; j < 5; j++)
@@ -1036,17 +1039,21 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
             if (lane == 0) hs.par[0][root] = (uint16_t)root;
         }
         __syncthreads();
-        // depth of every node: pointer jumping, ceil(log2(286)) = 9 rounds
-        for (uint32_t id = lane; id < 576; id += 64) {
-            const bool valid = id < 288 ? ((int)id < n && hs.key[id] != 0xFFFFFFFFu) : (id - 288) + 1 < used;
-            if (valid) hs.dep[0][id] = id == root ? 0 : 1;
+        // depth of every node (leaves 0..n-1, internal nodes 288 ..
+        // 288+used-2): pointer jumping; depths are below used, so
+        // ceil(log2(used)) rounds reach the root
+        const uint32_t nodes = (uint32_t)n + used - 1;  // leaf slots, then internal nodes
+        for (uint32_t t = lane; t < nodes; t += 64) {
+            const uint32_t id = t < (uint32_t)n ? t : 288 + (t - n);
+            if (id >= 288 || hs.key[id] != 0xFFFFFFFFu) hs.dep[0][id] = id == root ? 0 : 1;
         }
         __syncthreads();
         int cur = 0;
-        for (int r = 0; r < 9; r++) {
-            for (uint32_t id = lane; id < 576; id += 64) {
-                const bool valid = id < 288 ? ((int)id < n && hs.key[id] != 0xFFFFFFFFu) : (id - 288) + 1 < used;
-                if (!valid) continue;
+        const int rounds = 32 - __builtin_clz(used - 1);
+        for (int r = 0; r < rounds; r++) {
+            for (uint32_t t = lane; t < nodes; t += 64) {
+                const uint32_t id = t < (uint32_t)n ? t : 288 + (t - n);
+                if (id < 288 && hs.key[id] == 0xFFFFFFFFu) continue;
                 const uint32_t pa = hs.par[cur][id];
                 hs.dep[cur ^ 1][id] = hs.dep[cur][id] + hs.dep[cur][pa];
                 hs.par[cur ^ 1][id] = hs.par[cur][pa];
@@ -1322,14 +1329,23 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
         for (int d = 32; d >= 1; d >>= 1) body += __shfl_xor(body, d, 64);
         for (uint32_t i = lane; i < (pbits + 31) / 32; i += 64) w.prefix[(uint64_t)m * PREFIX_WORDS + i] = pbuf[i];
         // member CRC-32 from the segments' raw CRCs
+        // (crc(A||B) = crc(A) * x^(8|B|) ^ crc(B) on raw CRCs; segment s is
+        // followed by nseg-2-s full segments and the Lb bytes of the last)
         uint32_t crc = 0;
-        if (w.gzip) {
+        if (w.gzip && md.n) {
+            const uint32_t Lb = md.n - ((md.nseg - 1) << SEG_LOG2);  // 1 .. 4096
+            const uint32_t q = Lb >> 3;
+            const uint32_t XL = q ? gf2_mul(q < 512 ? c_crc_fold[511 - q] : c_segpow[1] /* Lb = 4096 */,
+                                            c_bytepow[Lb & 7u])
+                                  : c_bytepow[Lb & 7u];
             for (uint32_t s = lane; s < md.nseg; s += 64) {
-                const uint64_t end = min((uint64_t)(s + 1) << SEG_LOG2, (uint64_t)md.n);
-                crc ^= gf2_mul(w.seg_crc[md.seg_base + s], xpow8(md.n - end));
+                const uint32_t raw = w.seg_crc[md.seg_base + s];
+                crc ^= s + 1 == md.nseg ? raw : gf2_mul(raw, gf2_mul(c_segpow[md.nseg - 2 - s], XL));
             }
             for (int d = 32; d >= 1; d >>= 1) crc ^= __shfl_xor(crc, d, 64);
-            crc ^= gf2_mul(0xFFFFFFFFu, xpow8(md.n)) ^ 0xFFFFFFFFu;
+            crc ^= gf2_mul(0xFFFFFFFFu, gf2_mul(c_segpow[md.nseg - 1], XL)) ^ 0xFFFFFFFFu;
+        } else if (w.gzip) {
+            crc = 0;  // empty member: CRC-32 of nothing
         }
         if (lane == 0) {
             w.prefix_bits[m] = pbits;
@@ -1542,7 +1558,15 @@ hipError_t upload_constants() {
         return r;
     };
     for (uint32_t t = 0; t < 512; t++) fold[t] = pw(8ull * (511 - t));
+    static uint32_t segpow[4097];
+    uint32_t bytepow[8];
+    segpow[0] = 0x80000000u;
+    const uint32_t seg1 = pw(4096);
+    for (uint32_t j = 1; j <= 4096; j++) segpow[j] = mul(segpow[j - 1], seg1);
+    for (uint32_t b = 0; b < 8; b++) bytepow[b] = pw(b);
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_tab), tab, sizeof(tab));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_segpow), segpow, sizeof(segpow));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_bytepow), bytepow, sizeof(bytepow));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_fold), fold, sizeof(fold));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_xpow), xp, sizeof(xp));
     return e;
