@@ -304,7 +304,8 @@ def main():
     ap.add_argument("--mixed", action="store_true")
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=20000)
-    ap.add_argument("--serve-load", type=float, default=10.0)
+    ap.add_argument("--serve-load", type=float, nargs="+", default=[4.0, 8.0],
+                    help="offered loads (GB/s) of the serving runs, one run each")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy (y = x + 1, uint8) of MIB MiB three times; "
@@ -388,10 +389,11 @@ def main():
                           "k1_frac_hbm": round((rm["n_in"] + rm["n_out"]) / (rm["k1_ms"] / 1e3) / 1e9 / HBM_PEAK_GBPS, 5)}),
               flush=True)
     if args.serve:
-        log(f"serving: {args.serve_requests} requests at {args.serve_load} GB/s offered ...")
-        rs = serve(torch, ctx, args.serve_requests, args.serve_load)
-        print(json.dumps({"workload": "serving, Poisson arrivals, log-normal sizes (median 256 KiB, sigma 2)", **rs}),
-              flush=True)
+        for load in args.serve_load:
+            log(f"serving: {args.serve_requests} requests at {load} GB/s offered ...")
+            rs = serve(torch, ctx, args.serve_requests, load)
+            print(json.dumps({"workload": "serving, Poisson arrivals, log-normal sizes (median 256 KiB, sigma 2), "
+                                          "host buffers (PCIe both ways)", **rs}), flush=True)
     ctx.close()
 
 

@@ -42,6 +42,7 @@ using clk = std::chrono::steady_clock;
 struct Request {
     uint64_t tag = 0;
     std::vector<uint8_t> data;
+    uint64_t nbytes = 0;
     uint32_t nchunks = 0, done_chunks = 0;
     std::vector<std::vector<uint8_t>> chunk_out;
     std::vector<uint8_t> out;
@@ -104,31 +105,38 @@ struct gpz_server {
             if (!rc) rc = lane_batch_launch(ctx, ln, pl, ptrs.data(), lens.data());
             if (!rc) rc = lane_batch_collect(ln, pl, out, off, len);
             if (rc) (void)hipStreamSynchronize(ln->st);
+            // copy each chunk's members out without the lock (chunk_out slots
+            // are distinct per chunk and never resized while in flight)
+            if (!rc)
+                for (size_t i = 0; i < step.size(); i++)
+                    step[i].req->chunk_out[step[i].idx].assign(out.begin() + off[i], out.begin() + off[i] + len[i]);
             std::vector<Request *> completed;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 for (size_t i = 0; i < step.size(); i++) {
                     Request *r = step[i].req;
-                    if (rc) {
-                        r->status = rc;
-                    } else {
-                        r->chunk_out[step[i].idx].assign(out.begin() + off[i], out.begin() + off[i] + len[i]);
-                    }
+                    if (rc) r->status = rc;
                     if (++r->done_chunks == r->nchunks) completed.push_back(r);
                 }
+            }
+            // this worker finished these requests: concatenate outside the lock
+            for (Request *r : completed) {
+                size_t total = 0;
+                for (auto &c : r->chunk_out) total += c.size();
+                r->out.reserve(total);
+                for (auto &c : r->chunk_out) r->out.insert(r->out.end(), c.begin(), c.end());
+                r->chunk_out.clear();
+                r->chunk_out.shrink_to_fit();
+                r->data.clear();
+                r->data.shrink_to_fit();
+            }
+            if (!completed.empty()) {
+                std::lock_guard<std::mutex> lk(mu);
                 for (Request *r : completed) {
-                    size_t total = 0;
-                    for (auto &c : r->chunk_out) total += c.size();
-                    r->out.reserve(total);
-                    for (auto &c : r->chunk_out) r->out.insert(r->out.end(), c.begin(), c.end());
-                    r->chunk_out.clear();
-                    r->chunk_out.shrink_to_fit();
                     r->t_done = clk::now();
                     admitted -= r->charge;
                     n_done++;
-                    bytes_done += r->data.size();
-                    r->data.clear();
-                    r->data.shrink_to_fit();
+                    bytes_done += r->nbytes;
                     finished.push_back(r);
                 }
             }
@@ -193,6 +201,7 @@ int gpz_request_submit(gpz_server *s, const uint8_t *in, size_t len, uint64_t ta
     auto r = std::make_unique<Request>();
     r->tag = tag;
     r->data.assign(in, in + len);
+    r->nbytes = len;
     r->charge = charge;
     uint64_t cb = s->cfg.chunk_bytes;
     r->nchunks = len == 0 ? 1 : (uint32_t)((len + cb - 1) / cb);
