@@ -422,34 +422,43 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
 constexpr int K1_T = 1024;
 constexpr int WIN_WORDS = 9344;  // 37,376 bytes: 32 KiB back + 4 KiB segment + 258 + slack
 
-// Length of the match at cand (0 if the reference rejects it: cand >= p,
-// dist > window, or a mismatch in the first three bytes), capped at lim =
-// min(258, n - p).  own0 / own1 hold bytes p .. p+3 / p+4 .. p+7.  `hint`
-// carries what the previous position learnt at the same distance (see
-// hint_for): a length known to agree and, in bit 31, whether it is exact.
-__device__ __forceinline__ uint32_t cand_len(const uint32_t *win, uint32_t WB, uint32_t lim, uint32_t p,
-                                             uint32_t cand, uint32_t window, uint32_t own0, uint32_t own1,
-                                             uint32_t hint) {
+// A candidate resolved over its first 16 bytes (lz77_hash_lookup.wgsl:45-67:
+// rejected -- 0 -- when cand >= p, dist > window or the first three bytes
+// differ).  Returns the exact length, capped at lim = min(258, n - p), when it
+// is below 16 or lim <= 16; otherwise 16 | PEND: the length is at least 16 and
+// its exact value is left to ext_len.  own0 / own1 hold bytes p .. p+7.
+constexpr uint32_t PEND = 0x10000u;
+__device__ __forceinline__ uint32_t cand_short(const uint32_t *win, uint32_t WB, uint32_t lim, uint32_t p,
+                                               uint32_t cand, uint32_t window, uint32_t own0, uint32_t own1) {
     if (cand >= p || p - cand > window) return 0;
-    uint32_t len = hint & 0x1FFu;
-    if (hint >> 31) return len;
-    if (!len) {
-        // first eight bytes in one step: the three-byte check and the short matches
-        const uint32_t c = cand - WB, ic = c >> 2, sc = c & 3u;
-        const uint32_t w0 = win[ic], w1 = win[ic + 1], w2 = win[ic + 2];
-        const uint32_t x0 = own0 ^ __builtin_amdgcn_alignbyte(w1, w0, sc);
-        if (x0 & 0x00FFFFFFu) return 0;
-        if (x0) return 3;
-        const uint32_t x1 = own1 ^ __builtin_amdgcn_alignbyte(w2, w1, sc);
-        if (x1) return min(4 + ((uint32_t)__builtin_ctz(x1) >> 3), lim);
-        if (lim <= 8) return lim;
-        len = 8;
-    }
-    // 8 bytes per step: three dwords of each side realigned with alignbyte
-    // (re-read rather than carried, so the loop keeps no copies); one exit
-    const uint32_t ap = p - WB + len, ac = cand - WB + len;
-    const uint32_t sa = ap & 3u, sc = ac & 3u;
-    const uint32_t *pa = win + (ap >> 2), *pc = win + (ac >> 2);
+    const uint32_t c = cand - WB, ic = c >> 2, sc = c & 3u;
+    const uint32_t w0 = win[ic], w1 = win[ic + 1], w2 = win[ic + 2];
+    const uint32_t x0 = own0 ^ __builtin_amdgcn_alignbyte(w1, w0, sc);
+    if (x0 & 0x00FFFFFFu) return 0;
+    if (x0) return 3;
+    const uint32_t x1 = own1 ^ __builtin_amdgcn_alignbyte(w2, w1, sc);
+    if (x1) return min(4 + ((uint32_t)__builtin_ctz(x1) >> 3), lim);
+    if (lim <= 8) return lim;
+    // bytes 8 .. 15: own from the window, the candidate's dwords continue
+    const uint32_t a = p - WB + 8, ia = a >> 2, sa = a & 3u;
+    const uint32_t w3 = win[ic + 3], w4 = win[ic + 4];
+    const uint32_t a0 = win[ia], a1 = win[ia + 1], a2 = win[ia + 2];
+    const uint32_t y0 = __builtin_amdgcn_alignbyte(a1, a0, sa) ^ __builtin_amdgcn_alignbyte(w3, w2, sc);
+    const uint32_t y1 = __builtin_amdgcn_alignbyte(a2, a1, sa) ^ __builtin_amdgcn_alignbyte(w4, w3, sc);
+    if (y0 | y1)
+        return min(8 + (y0 ? (uint32_t)__builtin_ctz(y0) >> 3 : 4 + ((uint32_t)__builtin_ctz(y1) >> 3)), lim);
+    return lim <= 16 ? lim : (16u | PEND);
+}
+
+// Exact match length between window offsets a (position) and c (candidate)
+// that are known to agree over `len` bytes, capped at lim: 8 bytes per step,
+// three dwords of each side realigned with alignbyte, one exit.
+__device__ __forceinline__ uint32_t ext_len(const uint32_t *win, uint32_t a, uint32_t c, uint32_t len,
+                                            uint32_t lim) {
+    a += len;
+    c += len;
+    const uint32_t sa = a & 3u, sc = c & 3u;
+    const uint32_t *pa = win + (a >> 2), *pc = win + (c >> 2);
     uint32_t j = 0, x0, x1;
     do {
         x0 = __builtin_amdgcn_alignbyte(pa[j + 1], pa[j], sa) ^ __builtin_amdgcn_alignbyte(pc[j + 1], pc[j], sc);
@@ -460,18 +469,6 @@ __device__ __forceinline__ uint32_t cand_len(const uint32_t *win, uint32_t WB, u
     len += 4 * j - 8;
     len += x0 ? (uint32_t)__builtin_ctz(x0) >> 3 : (x1 ? 4 + ((uint32_t)__builtin_ctz(x1) >> 3) : 8);
     return len > lim ? lim : len;
-}
-
-// Result of one candidate at position p - 1 for the next position: length m
-// << 16 | distance, bit 31 set when the match ended at a mismatch (m below
-// that position's cap).  Bytes p-1 .. p-2+m agree with p-1-d .., so at p the
-// same distance agrees over m - 1 bytes, exactly m - 1 when bit 31 is set.
-__device__ __forceinline__ uint32_t hint_rec(uint32_t len, uint32_t dist, uint32_t lim) {
-    return len ? (len << 16) | dist | (len < lim ? 0x80000000u : 0u) : 0u;
-}
-__device__ __forceinline__ uint32_t hint_for(uint32_t d, uint32_t h) {
-    const uint32_t m = (h >> 16) & 0x1FFu;
-    return ((h & 0xFFFFu) == d && m >= 4) ? (m - 1) | (h & 0x80000000u) : 0u;
 }
 
 // The distinct hashes of the query segment get dense ids by rank: a 65536-bit
@@ -487,6 +484,7 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
     __shared__ uint16_t rnk[2048];   // set bits before each word of `bits`
     __shared__ uint32_t fl[SEG];     // per hash id: tag << 24 | first << 12 | last
     __shared__ uint32_t tmp[32];
+    __shared__ uint32_t kq[K1_T];    // per wave: 64-entry queue of pending candidates
     const uint32_t tid = threadIdx.x;
     const uint32_t G = uni(w.counts->nseg);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
@@ -596,44 +594,93 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
                 }
             }
             __syncthreads();
-            // evaluate segment s-k: gather both candidates' first bytes for all
-            // four positions, then apply the reference order position by
-            // position (arrays rotate so the loop body is not unrolled).  Each
-            // candidate's length at the previous position seeds the next one.
+            // evaluate segment s-k in the reference order per position: newest,
+            // then oldest while best < 16.  Pass 1 resolves every candidate over
+            // its first 16 bytes (position by position; the arrays rotate so the
+            // loop body is not unrolled).  A candidate whose 16 bytes agree is
+            // left pending: best is then >= 16 whatever its length, so the
+            // oldest rule is already decided, and a position has at most one
+            // pending candidate.  Pass 2 extends the pending candidates of the
+            // wave densely, 64 at a time, through a wave-private LDS queue.
             const uint32_t base = (s - k) << SEG_LOG2;
-            uint32_t fv[4];
+            uint32_t fv[4], pd[4] = {0, 0, 0, 0};
 #pragma unroll
             for (int i = 0; i < 4; i++)
                 fv[i] = (((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH) ? fl[qid[i]] : EMPTY;
-            uint32_t hn = 0, ho = 0;  // previous position's newest / oldest candidate (hint_rec)
 #pragma unroll 1
             for (uint32_t i = 0; i < 4; i++) {
                 const uint32_t v = fv[0];
-                uint32_t cs = st[0];
-                uint32_t rn = 0, ro = 0;
+                uint32_t cs = st[0], pend = 0;
                 if ((v >> 24) == k) {
                     const uint32_t p = S + x0 + i;
                     const uint32_t lim = min(n - p, MAX_MATCH);
                     const uint32_t own0 = __builtin_amdgcn_alignbyte(own_hi, own_lo, i);
                     const uint32_t own1 = __builtin_amdgcn_alignbyte(own_w2, own_hi, i);
                     const uint32_t cn = base + (v & 0xFFFu), co = base + ((v >> 12) & 0xFFFu);
-                    const uint32_t dn = p - cn, dd = p - co;
-                    const uint32_t ln = cand_len(win, WB, lim, p, cn, w.window, own0, own1, hint_for(dn, hn));
-                    rn = hint_rec(ln, dn, lim);
-                    if (ln > (cs >> 16)) cs = (ln << 16) | dn;
-                    // the oldest entry is tried only while best < 16
-                    // (lz77_hash_lookup.wgsl:102); equal to the newest it cannot
-                    // give a strictly longer match
-                    if ((cs >> 16) < OLDEST_SKIP && co != cn) {
-                        const uint32_t lo = cand_len(win, WB, lim, p, co, w.window, own0, own1, hint_for(dd, ho));
-                        ro = hint_rec(lo, dd, lim);
-                        if (lo > (cs >> 16)) cs = (lo << 16) | dd;
+                    const uint32_t ln = cand_short(win, WB, lim, p, cn, w.window, own0, own1);
+                    if (ln & PEND) {
+                        pend = 0x80000000u | (cn - WB);
+                    } else {
+                        if (ln > (cs >> 16)) cs = (ln << 16) | (p - cn);
+                        // the oldest entry is tried only while best < 16
+                        // (lz77_hash_lookup.wgsl:102); equal to the newest it
+                        // cannot give a strictly longer match
+                        if ((cs >> 16) < OLDEST_SKIP && co != cn) {
+                            const uint32_t lo = cand_short(win, WB, lim, p, co, w.window, own0, own1);
+                            if (lo & PEND) pend = 0xC0000000u | (co - WB);
+                            else if (lo > (cs >> 16)) cs = (lo << 16) | (p - co);
+                        }
                     }
                 }
-                hn = rn;
-                ho = ro;
                 fv[0] = fv[1]; fv[1] = fv[2]; fv[2] = fv[3];
                 st[0] = st[1]; st[1] = st[2]; st[2] = st[3]; st[3] = cs;
+                pd[0] = pd[1]; pd[1] = pd[2]; pd[2] = pd[3]; pd[3] = pend;
+            }
+            {
+                // pass 2 (wave-local: no workgroup barrier, the trip count differs per wave)
+                const uint32_t lane = tid & 63u;
+                const uint64_t lt = (1ull << lane) - 1;
+                uint32_t *q = kq + (tid & ~63u);
+                uint64_t bm[4];
+                uint32_t bs[4], T = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    bm[i] = __ballot(pd[i] != 0);
+                    bs[i] = T;
+                    T += __popcll(bm[i]);
+                }
+                for (uint32_t r0 = 0; r0 < T; r0 += 64) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const uint32_t idx = bs[i] + __popcll(bm[i] & lt) - r0;
+                        if (pd[i] && idx < 64) q[idx] = (x0 + i) | ((pd[i] & 0xFFFFu) << 12);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    if (r0 + lane < T) {
+                        const uint32_t it = q[lane];
+                        const uint32_t x = it & 0xFFFu;
+                        const uint32_t lim = min(n - (S + x), MAX_MATCH);
+                        q[lane] = ext_len(win, S - WB + x, it >> 12, 16, lim);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const uint32_t idx = bs[i] + __popcll(bm[i] & lt) - r0;
+                        if (pd[i] && idx < 64) {
+                            const uint32_t len = q[idx];
+                            const uint32_t d = S + x0 + i - (WB + (pd[i] & 0xFFFFu));
+                            // a pending oldest is strictly longer than best (< 16)
+                            if (((pd[i] >> 30) & 1u) || len > (st[i] >> 16)) st[i] = (len << 16) | d;
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
             }
             int active = 0;
 #pragma unroll
