@@ -419,6 +419,59 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
 // then oldest while best < 16, stopping at 258 (:96-110).  65 KB of LDS: two
 // workgroups per CU.
 
+// The walk summary of segment g from its tokens in LDS (tokL[0..L)); 1024
+// threads.  Ea/Ca are two ping-pong pairs of 4096 u16.  Ends before the
+// caller's closing barrier.
+__device__ __forceinline__ void seg_summary(const Work &w, uint32_t g, uint32_t L, bool last_seg, bool any_match,
+                                            const uint32_t *tokL, uint16_t (*Ea)[SEG], uint16_t (*Ca)[SEG]) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const bool absorb_last = !w.strict && !last_seg;
+    uint32_t *out = w.wsum + (uint64_t)g * SPAN;
+    // the last position is a self-loop only when it holds a match
+    const uint32_t absorb = (absorb_last && (tokL[L - 1] >> 16) != 0) ? L - 1 : 0xFFFFu;
+    if (!any_match) {
+        // no matches: every position is walked
+        for (uint32_t x = tid; x < L && x < SPAN; x += 1024) out[x] = L | ((L - x) << 16);
+        return;
+    }
+    // level 1: 16 waves x 4 blocks; per position the exit from its 64-position
+    // block and the tokens emitted inside it
+    for (uint32_t b = wid; b < 64; b += 16) {
+        const uint32_t bs = b * 64;
+        if (bs >= L) break;
+        const uint32_t x = bs + lane;
+        const bool valid = x < L;
+        bool dem;
+        const uint32_t J = valid ? succ_of(tokL, x, L, false, w.strict, absorb_last, dem) : 0;
+        uint64_t M;
+        const uint32_t ex = block_walk(J, bs, L, valid, valid && x == absorb, M);
+        if (valid) {
+            Ea[0][x] = (uint16_t)ex;
+            Ca[0][x] = (uint16_t)__popcll(M);
+        }
+    }
+    __syncthreads();
+    // level 2: pointer doubling over the block exits.  Every exit lies in a
+    // later block, so 64 hops (6 rounds) reach the segment end or the
+    // absorbing last position (its own exit, count 0).
+    uint32_t cur = 0;
+    for (int r = 0; r < 6; r++) {
+        for (uint32_t x = tid; x < L; x += 1024) {
+            const uint32_t y = Ea[cur][x];
+            uint32_t e2 = y, c2 = Ca[cur][x];
+            if (y < L) {
+                e2 = Ea[cur][y];
+                c2 += Ca[cur][y];
+            }
+            Ea[cur ^ 1][x] = (uint16_t)e2;
+            Ca[cur ^ 1][x] = (uint16_t)c2;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (uint32_t x = tid; x < L && x < SPAN; x += 1024) out[x] = Ea[cur][x] | ((uint32_t)Ca[cur][x] << 16);
+}
+
 constexpr int K1_T = 1024;
 constexpr int WIN_WORDS = 9344;  // 37,376 bytes: 32 KiB back + 4 KiB segment + 258 + slack
 
@@ -703,7 +756,22 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
             for (int i = 0; i < 4; i++)
                 if (x0 + i < L) tg[i] = t4[i];
         }
-        (void)do_summary;
+        if (do_summary) {
+            // the segment's walk summary from the same tokens, in the LDS the
+            // match search no longer needs: tokens over the window, the two
+            // exit arrays after them, the two count arrays over fl
+            __syncthreads();
+            uint32_t *tokL = win;
+            int anym = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                tokL[x0 + i] = t4[i];
+                anym |= (x0 + i < L) && (t4[i] >> 16) != 0;
+            }
+            const int any_match = __syncthreads_or(anym);
+            seg_summary(w, g, L, E == n, any_match != 0, tokL, reinterpret_cast<uint16_t (*)[SEG]>(win + SEG),
+                        reinterpret_cast<uint16_t (*)[SEG]>(fl));
+        }
         __syncthreads();
     }
 }
@@ -715,18 +783,16 @@ __global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) 
 // first token; that position is a self-loop here and k_chain resolves it.
 constexpr int KW_T = 1024;
 
+
 __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
     __shared__ uint32_t tokL[SEG];
     __shared__ uint16_t Ea[2][SEG], Ca[2][SEG];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t tid = threadIdx.x;
     const uint32_t G = uni(w.counts->nseg);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
         const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
-        const bool last_seg = E == n;
-        const bool absorb_last = !w.strict && !last_seg;
-        uint32_t *out = w.wsum + (uint64_t)g * SPAN;
         int anym = 0;
         for (uint32_t x = tid; x < L; x += KW_T) {
             const uint32_t t = w.tok[md.pos_base + S + x];
@@ -734,50 +800,7 @@ __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
             anym |= (t >> 16) != 0;
         }
         const int any_match = __syncthreads_or(anym);
-        // the last position is a self-loop only when it holds a match
-        const uint32_t absorb = (absorb_last && (tokL[L - 1] >> 16) != 0) ? L - 1 : 0xFFFFu;
-        if (!any_match) {
-            // no matches: every position is walked
-            for (uint32_t x = tid; x < L && x < SPAN; x += KW_T) out[x] = L | ((L - x) << 16);
-            __syncthreads();
-            continue;
-        }
-        // level 1: 16 waves x 4 blocks; per position the exit from its
-        // 64-position block and the tokens emitted inside it
-        for (uint32_t b = wid; b < 64; b += KW_T / 64) {
-            const uint32_t bs = b * 64;
-            if (bs >= L) break;
-            const uint32_t x = bs + lane;
-            const bool valid = x < L;
-            bool dem;
-            const uint32_t J = valid ? succ_of(tokL, x, L, false, w.strict, absorb_last, dem) : 0;
-            uint64_t M;
-            const uint32_t ex = block_walk(J, bs, L, valid, valid && x == absorb, M);
-            if (valid) {
-                Ea[0][x] = (uint16_t)ex;
-                Ca[0][x] = (uint16_t)__popcll(M);
-            }
-        }
-        __syncthreads();
-        // level 2: pointer doubling over the block exits.  Every exit lies in
-        // a later block, so 64 hops (6 rounds) reach the segment end or the
-        // absorbing last position (its own exit, count 0).
-        uint32_t cur = 0;
-        for (int r = 0; r < 6; r++) {
-            for (uint32_t x = tid; x < L; x += KW_T) {
-                const uint32_t y = Ea[cur][x];
-                uint32_t e2 = y, c2 = Ca[cur][x];
-                if (y < L) {
-                    e2 = Ea[cur][y];
-                    c2 += Ca[cur][y];
-                }
-                Ea[cur ^ 1][x] = (uint16_t)e2;
-                Ca[cur ^ 1][x] = (uint16_t)c2;
-            }
-            __syncthreads();
-            cur ^= 1;
-        }
-        for (uint32_t x = tid; x < L && x < SPAN; x += KW_T) out[x] = Ea[cur][x] | ((uint32_t)Ca[cur][x] << 16);
+        seg_summary(w, g, L, E == n, any_match != 0, tokL, Ea, Ca);
         __syncthreads();
     }
 }

@@ -278,9 +278,8 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
         default:  // MODE_FULL
             launch_seg_table(w, g.k0, st);
             if (ev) hipEventRecord(ev[5], st);
-            launch_match_find(w, g.k1, 0, st);
+            launch_match_find(w, g.k1, 1, st);  // with the walk summary
             if (ev) hipEventRecord(ev[1], st);
-            launch_walk_summary(w, g.k3, st);
             launch_chain(w, g.chain, st);
             launch_walk_emit(w, g.k3, st);
             launch_huffman(w, g.huff, st);
