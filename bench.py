@@ -92,7 +92,7 @@ def fetch(torch, d_out, off, ln):
     return d_out[int(off):int(off) + int(ln)].cpu().numpy().tobytes()
 
 
-def plain_and_batched(torch, ctx, data: np.ndarray, sizes, steps, warmup, check_oracle=2):
+def plain_and_batched(torch, ctx, data: np.ndarray, sizes, steps, warmup, check_oracle=2, autotune=True):
     """Returns dict with plain / batched timings for chunks `sizes` of `data`."""
     offs, total_in = layout(sizes)
     host = np.zeros(total_in + 256, np.uint8)
@@ -120,7 +120,17 @@ def plain_and_batched(torch, ctx, data: np.ndarray, sizes, steps, warmup, check_
         for p in plans:
             p.sync()
 
-    # batched form: one plan over every chunk, graph replay, persistent grids
+    # batched form: autotuned launch shapes, one plan over every chunk, graph
+    # replay, persistent grids
+    tune = None
+    if autotune:
+        probe = DevicePlan(ctx, list(zip(offs, sizes)), gpzip_amd.GPZ_PLAN_PERSISTENT)
+        d_probe = torch.zeros(probe.out_cap + 256, dtype=torch.uint8, device="cuda:0")
+        probe.close()
+        tune, t_ms, t_def = ctx.autotune(list(zip(offs, sizes)), base_in, d_probe.data_ptr())
+        del d_probe
+        ctx.set_tune(tune)
+        log(f"  autotune: {tune.as_dict()}  pipeline {t_def:.3f} -> {t_ms:.3f} ms")
     bplan = DevicePlan(ctx, list(zip(offs, sizes)), gpzip_amd.GPZ_PLAN_GRAPH | gpzip_amd.GPZ_PLAN_PERSISTENT)
     d_out_b = torch.zeros(bplan.out_cap + 256, dtype=torch.uint8, device="cuda:0")
 
@@ -181,6 +191,7 @@ def plain_and_batched(torch, ctx, data: np.ndarray, sizes, steps, warmup, check_
         k1.append(a_)
         emit.append(e_)
         k0.append(z_)
+    res["tune"] = tune.as_dict() if tune is not None else None
     res["k1_ms"] = float(np.mean(k1))
     res["k0_ms"] = float(np.mean(k0))
     res["emit_ms"] = float(np.mean(emit))
@@ -361,7 +372,7 @@ def main():
         "hbm_fraction_of_value": round(r["plain"]["gbps"] / HBM_PEAK_GBPS, 5),
         "batched": {"value": round(r["batched"]["gbps"], 3), "unit": "GB/s",
                     "ms_per_step": round(r["batched"]["s_per_step"] * 1e3, 3),
-                    "vs_plain": round(r["batched"]["gbps"] / r["plain"]["gbps"], 3)},
+                    "vs_plain": round(r["batched"]["gbps"] / r["plain"]["gbps"], 3), "autotune": r["tune"]},
         "roofline": {"bound": "hbm", "kernel": "k_match_find", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
                      "traffic": profiled_traffic(r["n_in"]), "alg_bytes_per_launch": alg_bytes,

@@ -26,7 +26,7 @@ ABI_SYMBOLS = [
     "gpz_match_find", "gpz_walk", "gpz_encode_block", "gpz_gzip_member",
     "gpz_batch_submit", "gpz_batch_wait", "gpz_batch_result", "gpz_batch_free",
     "gpz_plan_create", "gpz_plan_out_capacity", "gpz_plan_run", "gpz_plan_sync", "gpz_plan_result",
-    "gpz_plan_timing", "gpz_plan_destroy",
+    "gpz_plan_timing", "gpz_plan_destroy", "gpz_set_tune", "gpz_get_tune", "gpz_autotune",
     "gpz_server_config_default", "gpz_server_start", "gpz_request_submit", "gpz_request_next",
     "gpz_request_release", "gpz_server_stats", "gpz_server_stop",
 ]
@@ -48,7 +48,12 @@ class Desc(C.Structure):
 
 
 class Tune(C.Structure):
-    _fields_ = [("k1_grid", C.c_uint32), ("k3_grid", C.c_uint32), ("emit_grid", C.c_uint32), ("reserved", C.c_uint32)]
+    _fields_ = [("k0_grid", C.c_uint32), ("k1_grid", C.c_uint32), ("k3_grid", C.c_uint32), ("emit_grid", C.c_uint32),
+                ("k1_variant", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+    def as_dict(self) -> dict:
+        return {"k0_grid": self.k0_grid, "k1_grid": self.k1_grid, "k3_grid": self.k3_grid,
+                "emit_grid": self.emit_grid, "k1_variant": self.k1_variant}
 
 
 class ServerConfig(C.Structure):
@@ -97,6 +102,9 @@ def load_library() -> C.CDLL:
         "gpz_plan_sync": (C.c_int, [vp]),
         "gpz_plan_result": (C.c_int, [vp, u64p, u64p, u64p]),
         "gpz_plan_timing": (C.c_int, [vp, C.POINTER(C.c_float), C.c_size_t]),
+        "gpz_set_tune": (C.c_int, [vp, C.POINTER(Tune)]),
+        "gpz_get_tune": (C.c_int, [vp, C.POINTER(Tune)]),
+        "gpz_autotune": (C.c_int, [vp, C.POINTER(Desc), C.c_size_t, vp, vp, C.POINTER(Tune), C.POINTER(C.c_float)]),
         "gpz_plan_destroy": (None, [vp]),
         "gpz_server_config_default": (None, [C.POINTER(ServerConfig)]),
         "gpz_server_start": (C.c_int, [vp, C.POINTER(ServerConfig), C.POINTER(vp)]),
@@ -246,6 +254,26 @@ class Context:
             return res
         finally:
             self.L.gpz_batch_free(b)
+
+    def set_tune(self, tune: "Tune | None") -> None:
+        """Launch shapes for every batched launch of this context (None: built-in)."""
+        _check(self.L.gpz_set_tune(self.h, C.byref(tune) if tune is not None else None))
+
+    def get_tune(self) -> Tune:
+        t = Tune()
+        _check(self.L.gpz_get_tune(self.h, C.byref(t)))
+        return t
+
+    def autotune(self, descs, d_in: int, d_out: int) -> tuple[Tune, float, float]:
+        """Tune on a device-resident batch: descs = [(offset, len)], d_in / d_out
+        device pointers as for gpz_plan_run.  Returns (tune, tuned ms, built-in ms)."""
+        arr = (Desc * len(descs))()
+        for i, (o, n) in enumerate(descs):
+            arr[i].offset, arr[i].len = o, n
+        t = Tune()
+        ms = (C.c_float * 2)()
+        _check(self.L.gpz_autotune(self.h, arr, len(descs), C.c_void_p(d_in), C.c_void_p(d_out), C.byref(t), ms))
+        return t, float(ms[0]), float(ms[1])
 
 
 class Server:

@@ -62,10 +62,12 @@ int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *con
     w.gzip = 1;
     w.strict = 0;
     Grids g = grids_for(ln->wsp.caps, ctx_cus(ctx));
-    if (!ln->graph || ln->graph_gen != ln->wsp.generation || ln->graph_in != ln->din.p) {
+    apply_tune(g, ctx_tune(ctx));
+    if (!ln->graph || ln->graph_gen != ln->wsp.generation || ln->graph_in != ln->din.p || !(ln->graph_grids == g)) {
         if ((rc = capture_graph(w, g, ln->st, &ln->graph))) return rc;
         ln->graph_gen = ln->wsp.generation;
         ln->graph_in = ln->din.p;
+        ln->graph_grids = g;
     }
     HIPCHK(hipGraphLaunch(ln->graph, ln->st));
     return GPZ_OK;
@@ -220,11 +222,134 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
         p->grids.emit = p->grids.huff = p->grids.chain = std::max<uint32_t>(1, p->pl.M);
     }
     if (tune) {
-        if (tune->k1_grid) p->grids.k1 = tune->k1_grid;
-        if (tune->k3_grid) p->grids.k3 = tune->k3_grid;
-        if (tune->emit_grid) p->grids.emit = tune->emit_grid;
+        if (tune->k1_variant > 2) {
+            delete p;
+            return fail(GPZ_E_INVAL, "k1_variant must be 0, 1 or 2");
+        }
+        apply_tune(p->grids, *tune);
+    } else if (flags & GPZ_PLAN_PERSISTENT) {
+        apply_tune(p->grids, ctx_tune(ctx));
     }
     *out = p;
+    return GPZ_OK;
+}
+
+// Autotuner.  Kernel groups are timed one at a time on the caller's batch
+// (their times are independent: each runs alone on the stream), each over a
+// short candidate list, keeping the others at the best found so far.
+int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in, void *d_out, gpz_tune *out,
+                 float *ms) {
+    clear_error();
+    if (!ctx || !out || !d_in || !d_out || (n && !descs)) return fail(GPZ_E_INVAL, "bad argument");
+    gpz_plan *p = nullptr;
+    int rc = gpz_plan_create(ctx, descs, n, GPZ_PLAN_PERSISTENT, nullptr, &p);
+    if (rc) return rc;
+    const uint32_t cu = (uint32_t)ctx_cus(ctx);
+    const uint32_t G = std::max<uint32_t>(1, p->pl.G), M = std::max<uint32_t>(1, p->pl.M);
+    auto cap = [](uint64_t v, uint32_t hi) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(v, hi)); };
+    const Grids base = p->grids;
+    // time one configuration: median of three runs of the event interval [a, b)
+    auto timed = [&](const Grids &g, int a, int b, float *med) -> int {
+        p->grids = g;
+        float v[3];
+        for (int r = 0; r < 4; r++) {
+            int e = gpz_plan_run(p, d_in, d_out);
+            if (!e) e = gpz_plan_sync(p);
+            if (e) return e;
+            if (r == 0) continue;  // warm-up
+            float t = 0;
+            if (hipEventElapsedTime(&t, p->lane.ev[a], p->lane.ev[b]) != hipSuccess)
+                return fail(GPZ_E_DEVICE, "event timing failed");
+            v[r - 1] = t;
+        }
+        std::sort(v, v + 3);
+        *med = v[1];
+        return GPZ_OK;
+    };
+    // event order in enqueue: 0 start, 5 after seg_table, 1 after match-find,
+    // 3 before emit, 4 after emit, 2 end
+    Grids best = base;
+    float t0 = 0;
+    if ((rc = timed(base, 0, 2, &t0))) {
+        gpz_plan_destroy(p);
+        return rc;
+    }
+    struct Cand {
+        uint32_t grid, variant;
+    };
+    {  // match-find: occupancy variant x persistent grid
+        const Cand cs[] = {{cap((uint64_t)cu * 2, G), 0}, {cap((uint64_t)cu * 4, G), 0}, {cap((uint64_t)cu * 8, G), 0},
+                           {G, 0},  {cap(cu, G), 1}, {cap((uint64_t)cu * 2, G), 1}, {cap((uint64_t)cu * 8, G), 1}};
+        float bt = 1e30f;
+        for (const Cand &c : cs) {
+            Grids g = best;
+            g.k1 = c.grid;
+            g.k1v = c.variant;
+            float t;
+            if ((rc = timed(g, 5, 1, &t))) break;
+            if (t < bt) {
+                bt = t;
+                best.k1 = g.k1;
+                best.k1v = g.k1v;
+            }
+        }
+    }
+    if (!rc) {  // segment tables
+        const uint32_t cs[] = {cap((uint64_t)cu * 3, G), cap((uint64_t)cu * 6, G), cap((uint64_t)cu * 12, G), G};
+        float bt = 1e30f;
+        for (uint32_t c : cs) {
+            Grids g = best;
+            g.k0 = c;
+            float t;
+            if ((rc = timed(g, 0, 5, &t))) break;
+            if (t < bt) {
+                bt = t;
+                best.k0 = c;
+            }
+        }
+    }
+    if (!rc) {  // walk / token emission (with chain, Huffman and scan in the interval)
+        const uint32_t cs[] = {cap((uint64_t)cu * 2, G), cap((uint64_t)cu * 4, G), cap((uint64_t)cu * 8, G), G};
+        float bt = 1e30f;
+        for (uint32_t c : cs) {
+            Grids g = best;
+            g.k3 = c;
+            float t;
+            if ((rc = timed(g, 1, 3, &t))) break;
+            if (t < bt) {
+                bt = t;
+                best.k3 = c;
+            }
+        }
+    }
+    if (!rc) {  // bit emission
+        const uint32_t cs[] = {cap(cu, M), cap((uint64_t)cu * 2, M), cap((uint64_t)cu * 4, M), M};
+        float bt = 1e30f;
+        for (uint32_t c : cs) {
+            Grids g = best;
+            g.emit = c;
+            float t;
+            if ((rc = timed(g, 3, 4, &t))) break;
+            if (t < bt) {
+                bt = t;
+                best.emit = c;
+            }
+        }
+    }
+    float t1 = 0;
+    if (!rc) rc = timed(best, 0, 2, &t1);
+    gpz_plan_destroy(p);
+    if (rc) return rc;
+    *out = gpz_tune{};
+    out->k0_grid = best.k0;
+    out->k1_grid = best.k1;
+    out->k3_grid = best.k3;
+    out->emit_grid = best.emit;
+    out->k1_variant = best.k1v + 1;
+    if (ms) {
+        ms[0] = t1;
+        ms[1] = t0;
+    }
     return GPZ_OK;
 }
 

@@ -531,7 +531,10 @@ __device__ __forceinline__ uint32_t hash_rank(const uint32_t *bits, const uint16
     return rnk[h >> 5] + __popc(bits[h >> 5] & ((1u << (h & 31)) - 1u));
 }
 
-__global__ __launch_bounds__(K1_T, 8) void k_match_find(Work w, int do_summary) {
+// MINB: 8 gives two resident workgroups per CU (64 VGPRs, some spills), 4
+// gives one (128 VGPRs); the autotuner picks.
+template <int MINB>
+__global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summary) {
     __shared__ __align__(16) uint32_t win[WIN_WORDS];
     __shared__ uint32_t bits[2048];  // which hashes occur in the segment
     __shared__ uint16_t rnk[2048];   // set bits before each word of `bits`
@@ -1645,8 +1648,11 @@ hipError_t upload_constants() {
 void launch_seg_table(const Work &w, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_seg_table, dim3(grid), dim3(K0_T), 0, st, w);
 }
-void launch_match_find(const Work &w, uint32_t grid, int summary, hipStream_t st) {
-    hipLaunchKernelGGL(k_match_find, dim3(grid), dim3(K1_T), 0, st, w, summary);
+void launch_match_find(const Work &w, uint32_t grid, int summary, int variant, hipStream_t st) {
+    if (variant == 1)
+        hipLaunchKernelGGL(k_match_find<4>, dim3(grid), dim3(K1_T), 0, st, w, summary);
+    else
+        hipLaunchKernelGGL(k_match_find<8>, dim3(grid), dim3(K1_T), 0, st, w, summary);
 }
 void launch_walk_summary(const Work &w, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_walk_summary, dim3(grid), dim3(KW_T), 0, st, w);

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "gpz_internal.h"
+#include "gpzip_hip.h"
 
 struct gpz_ctx;
 
@@ -46,9 +47,16 @@ struct Layout {
 
 struct Grids {
     uint32_t k0 = 1, k1 = 1, k3 = 1, chain = 1, huff = 1, emit = 1;
+    uint32_t k1v = 0;  // match-find variant: 0 two workgroups per CU, 1 one
+    bool operator==(const Grids &o) const {
+        return k0 == o.k0 && k1 == o.k1 && k3 == o.k3 && chain == o.chain && huff == o.huff && emit == o.emit &&
+               k1v == o.k1v;
+    }
 };
 
 Grids grids_for(const Caps &c, int n_cu);
+// Override the persistent grids with the non-zero fields of a tuning record.
+void apply_tune(Grids &g, const gpz_tune &t);
 void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t *ev);
 
 }  // namespace gpz
@@ -91,6 +99,7 @@ struct Lane {
     hipGraphExec_t graph = nullptr;
     uint64_t graph_gen = 0;       // workspace generation the graph was captured for
     const void *graph_in = nullptr;
+    Grids graph_grids;            // launch shapes the graph was captured with
     uint32_t strict = 0;
     int init(int dev);
     ~Lane();
@@ -99,6 +108,7 @@ struct Lane {
 int ctx_device(gpz_ctx *c);
 int ctx_cus(gpz_ctx *c);
 uint32_t ctx_sub_chunk(gpz_ctx *c);
+const gpz_tune &ctx_tune(gpz_ctx *c);
 Lane *ctx_acquire(gpz_ctx *c);
 void ctx_release(gpz_ctx *c, Lane *l);
 int lane_run(gpz_ctx *ctx, Lane *ln, const Plan &pl, int mode, const uint8_t *host_in, const uint64_t *host_lens);

@@ -29,7 +29,7 @@ namespace gpz {
 
 hipError_t upload_constants();
 void launch_seg_table(const Work &w, uint32_t grid, hipStream_t st);
-void launch_match_find(const Work &w, uint32_t grid, int summary, hipStream_t st);
+void launch_match_find(const Work &w, uint32_t grid, int summary, int variant, hipStream_t st);
 void launch_walk_summary(const Work &w, uint32_t grid, hipStream_t st);
 void launch_chain(const Work &w, uint32_t grid, hipStream_t st);
 void launch_walk_emit(const Work &w, uint32_t grid, hipStream_t st);
@@ -242,15 +242,29 @@ int Workspace::upload_plan(const Plan &pl, hipStream_t st) {
     return GPZ_OK;
 }
 
+// Built-in launch shapes (what the autotuner picked on the binmix corpus):
+// the segment kernels with one workgroup per segment (the dispatcher balances
+// better than a persistent loop, and no registers stay live across
+// segments), the segment-table kernel persistent, member kernels one
+// workgroup per member.  Kernels still loop over the device-side counts, so a
+// graph captured for a capacity serves any smaller batch.
 Grids grids_for(const Caps &c, int n_cu) {
     Grids g;
-    g.k0 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.G, (uint64_t)n_cu * 8));
-    g.k1 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.G, (uint64_t)n_cu * 8));
-    g.k3 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.G, (uint64_t)n_cu * 8));
+    g.k0 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.G, (uint64_t)n_cu * 12));
+    g.k1 = (uint32_t)std::max<uint64_t>(1, c.G);
+    g.k3 = (uint32_t)std::max<uint64_t>(1, c.G);
     g.chain = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 16));
     g.huff = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 16));
-    g.emit = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 4));
+    g.emit = (uint32_t)std::max<uint64_t>(1, c.M);
     return g;
+}
+
+void apply_tune(Grids &g, const gpz_tune &t) {
+    if (t.k0_grid) g.k0 = t.k0_grid;
+    if (t.k1_grid) g.k1 = t.k1_grid;
+    if (t.k3_grid) g.k3 = t.k3_grid;
+    if (t.emit_grid) g.emit = t.emit_grid;
+    if (t.k1_variant == 1 || t.k1_variant == 2) g.k1v = t.k1_variant - 1;
 }
 
 // Enqueue the kernels of one run.  Events (if given) bracket the match-find
@@ -260,7 +274,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
     switch (mode) {
         case MODE_MATCH:
             launch_seg_table(w, g.k0, st);
-            launch_match_find(w, g.k1, 0, st);
+            launch_match_find(w, g.k1, 0, g.k1v, st);
             break;
         case MODE_WALK:
             launch_walk_summary(w, g.k1, st);
@@ -278,7 +292,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
         default:  // MODE_FULL
             launch_seg_table(w, g.k0, st);
             if (ev) hipEventRecord(ev[5], st);
-            launch_match_find(w, g.k1, 1, st);  // with the walk summary
+            launch_match_find(w, g.k1, 1, g.k1v, st);  // with the walk summary
             if (ev) hipEventRecord(ev[1], st);
             launch_chain(w, g.chain, st);
             launch_walk_emit(w, g.k3, st);
@@ -317,6 +331,7 @@ struct gpz_ctx {
     int device = 0;
     int n_cu = 256;
     gpz_config cfg{};
+    gpz_tune tune{};  // batched-form launch shapes (zero fields: built-in)
     char label[256] = {0};
     std::mutex mu;
     std::condition_variable cv;
@@ -343,6 +358,7 @@ namespace gpz {
 int ctx_device(gpz_ctx *c) { return c->device; }
 int ctx_cus(gpz_ctx *c) { return c->n_cu; }
 uint32_t ctx_sub_chunk(gpz_ctx *c) { return c->cfg.sub_chunk; }
+const gpz_tune &ctx_tune(gpz_ctx *c) { return c->tune; }
 Lane *ctx_acquire(gpz_ctx *c) { return c->acquire(); }
 void ctx_release(gpz_ctx *c, Lane *l) { c->release(l); }
 
@@ -404,6 +420,24 @@ int lane_fetch_chunk(Lane *ln, uint32_t c, uint8_t *out, size_t out_cap, size_t 
 // C ABI
 
 extern "C" {
+
+int gpz_set_tune(gpz_ctx *ctx, const gpz_tune *tune) {
+    clear_error();
+    if (!ctx) return fail(GPZ_E_INVAL, "null context");
+    if (tune && tune->k1_variant > 2) return fail(GPZ_E_INVAL, "k1_variant must be 0, 1 or 2");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (tune) ctx->tune = *tune;
+    else ctx->tune = gpz_tune{};
+    return GPZ_OK;
+}
+
+int gpz_get_tune(gpz_ctx *ctx, gpz_tune *tune) {
+    clear_error();
+    if (!ctx || !tune) return fail(GPZ_E_INVAL, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *tune = ctx->tune;
+    return GPZ_OK;
+}
 
 void gpz_config_default(gpz_config *cfg) {
     if (!cfg) return;

@@ -145,12 +145,31 @@ typedef struct gpz_plan gpz_plan;
 
 enum { GPZ_PLAN_GRAPH = 1u, GPZ_PLAN_PERSISTENT = 2u };
 
+/* Launch shape of the batched form.  Zero fields keep the built-in choice. */
 typedef struct gpz_tune {
-    uint32_t k1_grid;    /* persistent grid of the match-find kernel (0 = one WG per segment) */
-    uint32_t k3_grid;    /* persistent grid of the walk/compact kernel (0 = one WG per segment) */
-    uint32_t emit_grid;  /* persistent grid of the emit kernel (0 = one WG per member) */
-    uint32_t reserved;
+    uint32_t k0_grid;    /* persistent grid of the segment-table kernel */
+    uint32_t k1_grid;    /* persistent grid of the match-find kernel */
+    uint32_t k3_grid;    /* persistent grid of the walk/emit-tokens kernel */
+    uint32_t emit_grid;  /* persistent grid of the bit-emission kernel */
+    uint32_t k1_variant; /* match-find occupancy: 1 = two workgroups per CU (64 VGPRs),
+                            2 = one workgroup per CU (128 VGPRs); 0 = built-in */
+    uint32_t reserved[3];
 } gpz_tune;
+
+/* Tuning record used by every batched launch of this context (gpz_batch_*,
+ * the serving form, and gpz_plan_create with GPZ_PLAN_PERSISTENT and a NULL
+ * tune).  NULL restores the built-in grids.  No reference counterpart: the
+ * reference's batching constants are fixed (codec-gpu/src/gpu/batch.rs:31-35). */
+int gpz_set_tune(gpz_ctx *ctx, const gpz_tune *tune);
+int gpz_get_tune(gpz_ctx *ctx, gpz_tune *tune);
+
+/* Autotuner: times candidate launch shapes of every kernel group on the given
+ * device-resident batch (same arguments as gpz_plan_create + gpz_plan_run)
+ * and returns the fastest in *out.  ms[0] receives the pipeline time with the
+ * chosen shape, ms[1] with the built-in one (device time, HIP events).  Does
+ * not change the context's tuning record. */
+int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in, void *d_out, gpz_tune *out,
+                 float *ms);
 
 /* Plan for a fixed list of chunks laid out in a device buffer: workspace is
  * allocated once; gpz_plan_run may then be called repeatedly.  Chunk offsets

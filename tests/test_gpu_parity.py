@@ -163,6 +163,78 @@ def test_batch_equals_plain(ctx):
     assert ctx.batch(chunks) == res
 
 
+def test_autotune_and_tuned_shapes_equal_builtin(ctx):
+    """The autotuner returns a valid record; every launch shape it can pick
+    (and both match-find occupancy variants) gives the same bytes."""
+    import torch
+
+    kinds = ["text", "binmix", "log"]
+    sizes = [3 << 20, 1 << 20, 700001, 4096, 2 << 20]
+    chunks = [gpzip_amd.synth(kinds[i % 3], s, 90 + i) for i, s in enumerate(sizes)]
+    ref = [O.gzip_chunk(c) for c in chunks]
+    offs, o = [], 0
+    for c in chunks:
+        offs.append(o)
+        o += (c.size + 255) // 256 * 256
+    host = np.zeros(o + 256, np.uint8)
+    for off, c in zip(offs, chunks):
+        host[off:off + c.size] = c
+    d_in = torch.from_numpy(host).to("cuda:0")
+    descs = list(zip(offs, [c.size for c in chunks]))
+    arr = (gpzip_amd.Desc * len(descs))()
+    for i, (a, n) in enumerate(descs):
+        arr[i].offset, arr[i].len = a, n
+    L = ctx.L
+
+    def run(tune):
+        h = gpzip_amd.C.c_void_p()
+        t = gpzip_amd.C.byref(tune) if tune is not None else None
+        gpzip_amd._check(L.gpz_plan_create(ctx.h, arr, len(descs), gpzip_amd.GPZ_PLAN_PERSISTENT, t,
+                                           gpzip_amd.C.byref(h)))
+        try:
+            cap = L.gpz_plan_out_capacity(h)
+            d_out = torch.zeros(cap + 256, dtype=torch.uint8, device="cuda:0")
+            gpzip_amd._check(L.gpz_plan_run(h, gpzip_amd.C.c_void_p(d_in.data_ptr()),
+                                            gpzip_amd.C.c_void_p(d_out.data_ptr())))
+            gpzip_amd._check(L.gpz_plan_sync(h))
+            off = np.zeros(len(descs), np.uint64)
+            ln = np.zeros(len(descs), np.uint64)
+            tot = gpzip_amd.C.c_uint64()
+            gpzip_amd._check(L.gpz_plan_result(h, off.ctypes.data_as(gpzip_amd.u64p),
+                                               ln.ctypes.data_as(gpzip_amd.u64p), gpzip_amd.C.byref(tot)))
+            out = d_out.cpu().numpy()
+            return [out[int(a):int(a) + int(b)].tobytes() for a, b in zip(off, ln)]
+        finally:
+            L.gpz_plan_destroy(h)
+
+    say("autotune")
+    probe = torch.zeros(4 * o + (1 << 20), dtype=torch.uint8, device="cuda:0")
+    tune, t_ms, t_def = ctx.autotune(descs, d_in.data_ptr(), probe.data_ptr())
+    assert tune.k1_variant in (1, 2) and tune.k1_grid >= 1 and tune.k0_grid >= 1
+    assert tune.k3_grid >= 1 and tune.emit_grid >= 1 and t_ms > 0 and t_def > 0
+    assert run(tune) == ref
+    for v in (1, 2):
+        for g in (1, 7, 256, 100000):
+            t = gpzip_amd.Tune()
+            t.k0_grid, t.k1_grid, t.k3_grid, t.emit_grid, t.k1_variant = g, g, g, max(1, g // 3), v
+            say("shape", v, g)
+            assert run(t) == ref
+    # the context-wide record drives the batch API (graph re-captured on change)
+    t = gpzip_amd.Tune()
+    t.k1_grid, t.k1_variant = 64, 2
+    ctx.set_tune(t)
+    try:
+        assert ctx.get_tune().k1_variant == 2
+        assert ctx.batch(chunks) == ref
+    finally:
+        ctx.set_tune(None)
+    assert ctx.get_tune().k1_grid == 0
+    bad = gpzip_amd.Tune()
+    bad.k1_variant = 3
+    with pytest.raises(gpzip_amd.GpzError):
+        ctx.set_tune(bad)
+
+
 def test_server_alone_equals_mix(ctx):
     kinds = ["text", "binmix", "log"]
     reqs = [gpzip_amd.synth(kinds[i % 3], s, 70 + i)
