@@ -260,7 +260,7 @@ def mixed_sizes_workload(seed=3, count=4096):
     return np.concatenate(parts), [int(s) for s in sizes]
 
 
-def serve(torch, ctx, n_requests, load_gbps, seed=4):
+def serve(torch, ctx, n_requests, load_gbps, seed=4, **server_cfg):
     sizes = gpzip_amd.lognormal_sizes(seed, n_requests, 256 * 1024, 2.0, 1024, 64 << 20).astype(np.int64)
     kinds = ["text", "binmix", "log"]
     pool_n = (64 << 20) + (192 << 20)
@@ -268,7 +268,7 @@ def serve(torch, ctx, n_requests, load_gbps, seed=4):
     choose = gpzip_amd.uniform(seed + 1, n_requests)
     offs = (gpzip_amd.uniform(seed + 2, n_requests) * (pool_n - sizes)).astype(np.int64)
     gaps = -np.log(1.0 - gpzip_amd.uniform(seed + 3, n_requests)) * (sizes.mean() / (load_gbps * 1e9))
-    srv = gpzip_amd.Server(ctx, budget_bytes=64 * 10**9)
+    srv = gpzip_amd.Server(ctx, budget_bytes=64 * 10**9, **server_cfg)
     lat = []
     done = 0
     out_bytes = 0
@@ -315,7 +315,7 @@ def main():
     ap.add_argument("--mixed", action="store_true")
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=20000)
-    ap.add_argument("--serve-load", type=float, nargs="+", default=[4.0, 8.0],
+    ap.add_argument("--serve-load", type=float, nargs="+", default=[2.0, 4.0, 8.0, 12.0],
                     help="offered loads (GB/s) of the serving runs, one run each")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -279,13 +279,16 @@ class Context:
 class Server:
     """Serving form (gpz_server_*)."""
 
-    def __init__(self, ctx: Context, budget_bytes: int = 64 << 30, max_batch_chunks: int = 512,
-                 max_batch_bytes: int = 256 << 20, chunk_bytes: int = 2 << 20, streams: int = 3):
+    def __init__(self, ctx: Context, budget_bytes: int | None = None, max_batch_chunks: int | None = None,
+                 max_batch_bytes: int | None = None, chunk_bytes: int | None = None, streams: int | None = None):
+        """Fields left at None keep gpz_server_config_default's values."""
         self.L = ctx.L
         cfg = ServerConfig()
         self.L.gpz_server_config_default(C.byref(cfg))
-        cfg.budget_bytes, cfg.max_batch_chunks, cfg.max_batch_bytes = budget_bytes, max_batch_chunks, max_batch_bytes
-        cfg.chunk_bytes, cfg.streams = chunk_bytes, streams
+        for name, v in (("budget_bytes", budget_bytes), ("max_batch_chunks", max_batch_chunks),
+                        ("max_batch_bytes", max_batch_bytes), ("chunk_bytes", chunk_bytes), ("streams", streams)):
+            if v is not None:
+                setattr(cfg, name, v)
         h = C.c_void_p()
         _check(self.L.gpz_server_start(ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h

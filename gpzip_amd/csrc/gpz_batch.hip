@@ -73,8 +73,8 @@ int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *con
     return GPZ_OK;
 }
 
-int lane_batch_collect(Lane *ln, const Plan &pl, std::vector<uint8_t> &out, std::vector<uint64_t> &off,
-                       std::vector<uint64_t> &len) {
+int lane_batch_collect_pinned(Lane *ln, const Plan &pl, const uint8_t **data, std::vector<uint64_t> &off,
+                              std::vector<uint64_t> &len) {
     off.resize(pl.C);
     len.resize(pl.C);
     uint64_t total = 0;
@@ -89,7 +89,18 @@ int lane_batch_collect(Lane *ln, const Plan &pl, std::vector<uint8_t> &out, std:
         HIPCHK(hipMemcpyAsync(ln->hout.p, w.out, total, hipMemcpyDeviceToHost, ln->st));
         HIPCHK(hipStreamSynchronize(ln->st));
     }
-    out.assign((uint8_t *)ln->hout.p, (uint8_t *)ln->hout.p + total);
+    *data = (const uint8_t *)ln->hout.p;
+    return GPZ_OK;
+}
+
+int lane_batch_collect(Lane *ln, const Plan &pl, std::vector<uint8_t> &out, std::vector<uint64_t> &off,
+                       std::vector<uint64_t> &len) {
+    const uint8_t *d = nullptr;
+    int rc = lane_batch_collect_pinned(ln, pl, &d, off, len);
+    if (rc) return rc;
+    uint64_t total = 0;
+    for (size_t c = 0; c < pl.C; c++) total = std::max<uint64_t>(total, off[c] + len[c]);
+    out.assign(d, d + total);
     return GPZ_OK;
 }
 

@@ -120,5 +120,8 @@ int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *con
 // Wait for the step and copy every chunk's members to host.
 int lane_batch_collect(Lane *ln, const Plan &pl, std::vector<uint8_t> &out, std::vector<uint64_t> &off,
                        std::vector<uint64_t> &len);
+// Same, leaving the members in the lane's pinned staging (valid until the lane's next step).
+int lane_batch_collect_pinned(Lane *ln, const Plan &pl, const uint8_t **data, std::vector<uint64_t> &off,
+                              std::vector<uint64_t> &len);
 
 }  // namespace gpz

@@ -77,7 +77,6 @@ struct gpz_server {
         std::vector<ChunkRef> step;
         std::vector<const uint8_t *> ptrs;
         std::vector<uint64_t> lens, off, len;
-        std::vector<uint8_t> out;
         for (;;) {
             step.clear();
             {
@@ -103,13 +102,14 @@ struct gpz_server {
             Plan pl;
             int rc = build_plan(pl, lens.data(), nullptr, step.size(), ctx_sub_chunk(ctx), 1);
             if (!rc) rc = lane_batch_launch(ctx, ln, pl, ptrs.data(), lens.data());
-            if (!rc) rc = lane_batch_collect(ln, pl, out, off, len);
+            const uint8_t *res = nullptr;
+            if (!rc) rc = lane_batch_collect_pinned(ln, pl, &res, off, len);
             if (rc) (void)hipStreamSynchronize(ln->st);
             // copy each chunk's members out without the lock (chunk_out slots
             // are distinct per chunk and never resized while in flight)
             if (!rc)
                 for (size_t i = 0; i < step.size(); i++)
-                    step[i].req->chunk_out[step[i].idx].assign(out.begin() + off[i], out.begin() + off[i] + len[i]);
+                    step[i].req->chunk_out[step[i].idx].assign(res + off[i], res + off[i] + len[i]);
             std::vector<Request *> completed;
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -121,10 +121,14 @@ struct gpz_server {
             }
             // this worker finished these requests: concatenate outside the lock
             for (Request *r : completed) {
-                size_t total = 0;
-                for (auto &c : r->chunk_out) total += c.size();
-                r->out.reserve(total);
-                for (auto &c : r->chunk_out) r->out.insert(r->out.end(), c.begin(), c.end());
+                if (r->chunk_out.size() == 1) {
+                    r->out = std::move(r->chunk_out[0]);  // one chunk: no concatenation
+                } else {
+                    size_t total = 0;
+                    for (auto &c : r->chunk_out) total += c.size();
+                    r->out.reserve(total);
+                    for (auto &c : r->chunk_out) r->out.insert(r->out.end(), c.begin(), c.end());
+                }
                 r->chunk_out.clear();
                 r->chunk_out.shrink_to_fit();
                 r->data.clear();
@@ -153,10 +157,12 @@ extern "C" {
 void gpz_server_config_default(gpz_server_config *cfg) {
     if (!cfg) return;
     cfg->budget_bytes = 64ull << 30;
-    cfg->max_batch_chunks = 512;
-    cfg->max_batch_bytes = 256u << 20;
+    // a step of 32 MiB keeps ~8k segments in flight (the GPU stays full) while
+    // bounding the time a request waits behind a step to a few milliseconds
+    cfg->max_batch_chunks = 64;
+    cfg->max_batch_bytes = 32u << 20;
     cfg->chunk_bytes = 2u << 20;
-    cfg->streams = 3;
+    cfg->streams = 6;
 }
 
 int gpz_server_start(gpz_ctx *ctx, const gpz_server_config *cfg_in, gpz_server **out) {
