@@ -881,8 +881,29 @@ __global__ __launch_bounds__(64) void k_chain(Work w) {
 
 constexpr int K3_T = 1024;
 
+// Tokens of segment g for this thread (positions 4t .. 4t+3; 0 past the
+// segment end) and, on thread 0, the next segment's first token.  Issued one
+// segment ahead so the loads are in flight while the current one is walked.
+__device__ __forceinline__ void tok_prefetch(const Work &w, uint32_t G, uint32_t g, uint4 &q, uint32_t &tn) {
+    q = make_uint4(0, 0, 0, 0);
+    tn = 0;
+    if (g >= G) return;
+    const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
+    const uint32_t S = (g - md.seg_base) << SEG_LOG2, E = min(S + SEG, md.n), L = E - S;
+    const uint32_t *tg = w.tok + md.pos_base + S;
+    const uint32_t x = threadIdx.x * 4u;
+    if (x + 4 <= L) {
+        q = *reinterpret_cast<const uint4 *>(tg + x);
+    } else {
+        if (x < L) q.x = tg[x];
+        if (x + 1 < L) q.y = tg[x + 1];
+        if (x + 2 < L) q.z = tg[x + 2];
+    }
+    if (threadIdx.x == 0 && E < md.n) tn = tg[L];
+}
+
 __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
-    __shared__ uint32_t tokL[SEG + 4];
+    __shared__ __align__(16) uint32_t tokL[SEG + 4];
     __shared__ uint64_t Mv[SEG];     // per position: positions walked inside its block from it
     __shared__ uint16_t Ja[SEG];     // per position: exit from its block
     __shared__ uint64_t vb[64];      // walked positions of each block
@@ -892,18 +913,24 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint64_t lt = (1ull << lane) - 1;
     const uint32_t G = uni(w.counts->nseg);
+    uint4 q;
+    uint32_t tn;
+    tok_prefetch(w, G, xcd_block(), q, tn);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
         const uint32_t m = uni(w.seg_mem[g]);
         const MemberDesc md = load_member(w.mem, m);
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
         const uint32_t e = uni(w.seg_entry[g]);
-        if (e >= L) continue;  // the walk jumps over this segment (uniform)
+        if (e >= L) {  // the walk jumps over this segment (uniform)
+            tok_prefetch(w, G, g + gridDim.x, q, tn);
+            continue;
+        }
         const bool next_known = E < n;
         const uint32_t tokbase = w.seg_tokbase[g];
-        const uint32_t *tg = w.tok + md.pos_base + S;
-        for (uint32_t x = tid; x < L; x += K3_T) tokL[x] = tg[x];
-        if (tid == 0) tokL[L] = next_known ? tg[L] : 0;
+        reinterpret_cast<uint4 *>(tokL)[tid] = q;
+        if (tid == 0 && L == SEG) tokL[SEG] = tn;  // (a shorter segment is its member's last)
+        tok_prefetch(w, G, g + gridDim.x, q, tn);
         for (uint32_t i = tid; i < NSYM; i += K3_T) hist[i] = 0;
         if (tid < 64) vb[tid] = 0;
         __syncthreads();
