@@ -1027,26 +1027,24 @@ struct HuffScratch {
     uint16_t dep[2][576];
 };
 
-// Wave-uniform element t of a 5-register array spread over the lanes
-// (element t in lane t % 64 of register t / 64).
-__device__ __forceinline__ uint32_t rd5(const uint32_t (&a)[5], uint32_t t) {
-    const uint32_t l = t & 63u;
-    switch (t >> 6) {
-        case 0: return __builtin_amdgcn_readlane(a[0], l);
-        case 1: return __builtin_amdgcn_readlane(a[1], l);
-        case 2: return __builtin_amdgcn_readlane(a[2], l);
-        case 3: return __builtin_amdgcn_readlane(a[3], l);
-        default: return __builtin_amdgcn_readlane(a[4], l);
+// Register j (uniform, 0..4) of a 5-register array: element t of the array
+// lives in lane t % 64 of register t / 64.
+__device__ __forceinline__ uint32_t rd5_blk(const uint32_t (&a)[5], uint32_t j) {
+    switch (j) {
+        case 0: return a[0];
+        case 1: return a[1];
+        case 2: return a[2];
+        case 3: return a[3];
+        default: return a[4];
     }
 }
-__device__ __forceinline__ void wr5(uint32_t (&a)[5], uint32_t t, uint32_t v) {
-    const bool mine = (threadIdx.x & 63u) == (t & 63u);
-    switch (t >> 6) {
-        case 0: a[0] = mine ? v : a[0]; break;
-        case 1: a[1] = mine ? v : a[1]; break;
-        case 2: a[2] = mine ? v : a[2]; break;
-        case 3: a[3] = mine ? v : a[3]; break;
-        default: a[4] = mine ? v : a[4]; break;
+__device__ __forceinline__ void wr5_blk(uint32_t (&a)[5], uint32_t j, uint32_t v) {
+    switch (j) {
+        case 0: a[0] = v; break;
+        case 1: a[1] = v; break;
+        case 2: a[2] = v; break;
+        case 3: a[3] = v; break;
+        default: a[4] = v; break;
     }
 }
 
@@ -1113,30 +1111,49 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
             iq[j] = 0;
         }
         // merge: wave-uniform control, one pick per step from the leaf list
-        // or the internal FIFO
+        // or the internal FIFO.  All three cursors only move forward, so each
+        // works on one 64-element block held in a register and changes block
+        // at most four times: a pick is one readlane.
         {
-            uint32_t nl = 0, qh = 0;
-            uint32_t lkey = rd5(sk, 0);
-            for (uint32_t k = 0; k + 1 < used; k++) {
+            const uint32_t me = lane;
+            uint32_t nl = 0, qh = 0, k = 0;
+            uint32_t lblk = sk[0];   // leaf block of nl
+            uint32_t wblk = 0;       // internal block being written (k)
+            uint32_t rblk = 0;       // internal block being read (qh), when it is not wblk
+            uint32_t lkey = __builtin_amdgcn_readlane(lblk, 0);
+            for (; k + 1 < used; k++) {
                 uint32_t fsum = 0;
 #pragma unroll
                 for (int pick = 0; pick < 2; pick++) {
                     const bool have_int = qh < k;
-                    const uint32_t qf = have_int ? rd5(iq, qh) : 0u;
+                    uint32_t qf = 0;
+                    if (have_int) {
+                        const uint32_t src = ((qh ^ k) >> 6) ? rblk : wblk;
+                        qf = __builtin_amdgcn_readlane(src, qh & 63u);
+                    }
                     if (nl < used && (!have_int || (lkey >> 9) <= qf)) {
                         fsum += lkey >> 9;
-                        if (lane == 0) hs.par[0][lkey & 0x1FFu] = (uint16_t)(288 + k);
+                        if (me == 0) hs.par[0][lkey & 0x1FFu] = (uint16_t)(288 + k);
                         nl++;
-                        lkey = nl < used ? rd5(sk, nl) : 0xFFFFFFFFu;
+                        if ((nl & 63u) == 0) lblk = rd5_blk(sk, nl >> 6);
+                        lkey = nl < used ? __builtin_amdgcn_readlane(lblk, nl & 63u) : 0xFFFFFFFFu;
                     } else {
                         fsum += qf;
-                        if (lane == 0) hs.par[0][288 + qh] = (uint16_t)(288 + k);
+                        if (me == 0) hs.par[0][288 + qh] = (uint16_t)(288 + k);
                         qh++;
+                        // entering a block the writer has already left: fetch it
+                        if ((qh & 63u) == 0 && (qh >> 6) != (k >> 6)) rblk = rd5_blk(iq, qh >> 6);
                     }
                 }
-                wr5(iq, k, fsum);
+                wblk = me == (k & 63u) ? fsum : wblk;
+                if (((k + 1) & 63u) == 0) {
+                    // the writer leaves its block: keep it for the reader
+                    wr5_blk(iq, k >> 6, wblk);
+                    if ((qh >> 6) == (k >> 6)) rblk = wblk;
+                    wblk = 0;
+                }
             }
-            if (lane == 0) hs.par[0][root] = (uint16_t)root;
+            if (me == 0) hs.par[0][root] = (uint16_t)root;
         }
         __syncthreads();
         // depth of every node (leaves 0..n-1, internal nodes 288 ..
