@@ -1113,7 +1113,10 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
         // merge: wave-uniform control, one pick per step from the leaf list
         // or the internal FIFO.  All three cursors only move forward, so each
         // works on one 64-element block held in a register and changes block
-        // at most four times: a pick is one readlane.
+        // at most four times: a pick is one readlane.  Internal nodes carry
+        // freq << 9 | height, so the root's height -- the longest code --
+        // is known without a depth pass.
+        uint32_t root_h = 0;
         {
             const uint32_t me = lane;
             uint32_t nl = 0, qh = 0, k = 0;
@@ -1122,30 +1125,32 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
             uint32_t rblk = 0;       // internal block being read (qh), when it is not wblk
             uint32_t lkey = __builtin_amdgcn_readlane(lblk, 0);
             for (; k + 1 < used; k++) {
-                uint32_t fsum = 0;
+                uint32_t fsum = 0, hgt = 0;
 #pragma unroll
                 for (int pick = 0; pick < 2; pick++) {
                     const bool have_int = qh < k;
-                    uint32_t qf = 0;
+                    uint32_t qv = 0;
                     if (have_int) {
                         const uint32_t src = ((qh ^ k) >> 6) ? rblk : wblk;
-                        qf = __builtin_amdgcn_readlane(src, qh & 63u);
+                        qv = __builtin_amdgcn_readlane(src, qh & 63u);
                     }
-                    if (nl < used && (!have_int || (lkey >> 9) <= qf)) {
+                    if (nl < used && (!have_int || (lkey >> 9) <= (qv >> 9))) {
                         fsum += lkey >> 9;
                         if (me == 0) hs.par[0][lkey & 0x1FFu] = (uint16_t)(288 + k);
                         nl++;
                         if ((nl & 63u) == 0) lblk = rd5_blk(sk, nl >> 6);
                         lkey = nl < used ? __builtin_amdgcn_readlane(lblk, nl & 63u) : 0xFFFFFFFFu;
                     } else {
-                        fsum += qf;
+                        fsum += qv >> 9;
+                        hgt = max(hgt, qv & 0x1FFu);
                         if (me == 0) hs.par[0][288 + qh] = (uint16_t)(288 + k);
                         qh++;
                         // entering a block the writer has already left: fetch it
                         if ((qh & 63u) == 0 && (qh >> 6) != (k >> 6)) rblk = rd5_blk(iq, qh >> 6);
                     }
                 }
-                wblk = me == (k & 63u) ? fsum : wblk;
+                root_h = hgt + 1;
+                wblk = me == (k & 63u) ? (fsum << 9) | root_h : wblk;
                 if (((k + 1) & 63u) == 0) {
                     // the writer leaves its block: keep it for the reader
                     wr5_blk(iq, k >> 6, wblk);
@@ -1154,6 +1159,14 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
                 }
             }
             if (me == 0) hs.par[0][root] = (uint16_t)root;
+        }
+        if (root_h > (uint32_t)limit) {
+            // too long: halve every frequency > 1 and rebuild (huffman.rs:44-58)
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                if (fr[j] > 1) fr[j] >>= 1;
+            continue;
         }
         __syncthreads();
         // depth of every node (leaves 0..n-1, internal nodes 288 ..
@@ -1178,22 +1191,11 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
             __syncthreads();
             cur ^= 1;
         }
-        uint32_t mx = 0;
 #pragma unroll
         for (int j = 0; j < 5; j++)
-            if (fr[j]) mx = max(mx, (uint32_t)hs.dep[cur][lane + 64 * j]);
-        for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
-        if (mx <= (uint32_t)limit) {
-#pragma unroll
-            for (int j = 0; j < 5; j++)
-                if ((int)(lane + 64 * j) < n) lens[lane + 64 * j] = fr[j] ? (uint8_t)hs.dep[cur][lane + 64 * j] : 0;
-            __syncthreads();
-            return;
-        }
+            if ((int)(lane + 64 * j) < n) lens[lane + 64 * j] = fr[j] ? (uint8_t)hs.dep[cur][lane + 64 * j] : 0;
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 5; j++)
-            if (fr[j] > 1) fr[j] >>= 1;
+        return;
     }
 }
 
