@@ -187,6 +187,14 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_re
 __device__ __forceinline__ uint64_t uni64(uint64_t v) {
     return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v);
 }
+// One dword at a wave-uniform index through the scalar cache.  The wait is on
+// lgkmcnt, so vector loads still in flight (prefetches) are not waited for.
+__device__ __forceinline__ uint32_t sload_u32(const uint32_t *base, uint32_t i) {
+    const uint32_t *p = reinterpret_cast<const uint32_t *>(uni64(reinterpret_cast<uint64_t>(base + i)));
+    uint32_t r;
+    asm volatile("s_load_dword %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=s"(r) : "s"(p) : "memory");
+    return r;
+}
 // Member descriptor m, read by a whole workgroup (m must be uniform), with
 // one scalar load into scalar registers.  The compiler cannot prove the
 // descriptor array unwritten inside kernels that store to global memory, so
@@ -882,24 +890,37 @@ __global__ __launch_bounds__(64) void k_chain(Work w) {
 constexpr int K3_T = 1024;
 
 // Tokens of segment g for this thread (positions 4t .. 4t+3; 0 past the
-// segment end) and, on thread 0, the next segment's first token.  Issued one
-// segment ahead so the loads are in flight while the current one is walked.
-__device__ __forceinline__ void tok_prefetch(const Work &w, uint32_t G, uint32_t g, uint4 &q, uint32_t &tn) {
-    q = make_uint4(0, 0, 0, 0);
-    tn = 0;
+// segment end), the input bytes at those positions and, on thread 0, the next
+// segment's first token.  Issued one segment ahead so the loads are in flight
+// while the current one is walked; every wait before their use is a scalar
+// (lgkmcnt) one, which leaves them outstanding.
+struct SegPrefetch {
+    uint4 q;
+    uint32_t raw, tn;
+};
+__device__ __forceinline__ void tok_prefetch(const Work &w, uint32_t G, uint32_t g, SegPrefetch &f) {
+    f.q = make_uint4(0, 0, 0, 0);
+    f.raw = 0;
+    f.tn = 0;
     if (g >= G) return;
-    const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
+    const MemberDesc md = load_member(w.mem, sload_u32(w.seg_mem, g));
     const uint32_t S = (g - md.seg_base) << SEG_LOG2, E = min(S + SEG, md.n), L = E - S;
     const uint32_t *tg = w.tok + md.pos_base + S;
+    const uint8_t *ib = w.in + md.in_off + S;
     const uint32_t x = threadIdx.x * 4u;
     if (x + 4 <= L) {
-        q = *reinterpret_cast<const uint4 *>(tg + x);
+        f.q = *reinterpret_cast<const uint4 *>(tg + x);
+        // member starts are 16-byte aligned, segments 4 KiB apart
+        f.raw = *reinterpret_cast<const uint32_t *>(ib + x);
     } else {
-        if (x < L) q.x = tg[x];
-        if (x + 1 < L) q.y = tg[x + 1];
-        if (x + 2 < L) q.z = tg[x + 2];
+        if (x < L) f.q.x = tg[x];
+        if (x + 1 < L) f.q.y = tg[x + 1];
+        if (x + 2 < L) f.q.z = tg[x + 2];
+        if (x < L) f.raw = ib[x];
+        if (x + 1 < L) f.raw |= (uint32_t)ib[x + 1] << 8;
+        if (x + 2 < L) f.raw |= (uint32_t)ib[x + 2] << 16;
     }
-    if (threadIdx.x == 0 && E < md.n) tn = tg[L];
+    if (threadIdx.x == 0 && E < md.n) f.tn = tg[L];
 }
 
 __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
@@ -910,27 +931,31 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
     __shared__ uint64_t demb[64];    // demoted matches of each block (lazy rule)
     __shared__ uint32_t cntb[64];
     __shared__ uint32_t hist[NSYM];
+    __shared__ uint32_t rawL[SEG / 4];  // the segment's input bytes
     const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint64_t lt = (1ull << lane) - 1;
     const uint32_t G = uni(w.counts->nseg);
-    uint4 q;
-    uint32_t tn;
-    tok_prefetch(w, G, xcd_block(), q, tn);
+    SegPrefetch pf;
+    tok_prefetch(w, G, xcd_block(), pf);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
-        const uint32_t m = uni(w.seg_mem[g]);
+        const uint32_t m = sload_u32(w.seg_mem, g);
         const MemberDesc md = load_member(w.mem, m);
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
-        const uint32_t e = uni(w.seg_entry[g]);
+        const uint32_t e = sload_u32(w.seg_entry, g);
         if (e >= L) {  // the walk jumps over this segment (uniform)
-            tok_prefetch(w, G, g + gridDim.x, q, tn);
+            tok_prefetch(w, G, g + gridDim.x, pf);
             continue;
         }
         const bool next_known = E < n;
-        const uint32_t tokbase = w.seg_tokbase[g];
-        reinterpret_cast<uint4 *>(tokL)[tid] = q;
-        if (tid == 0 && L == SEG) tokL[SEG] = tn;  // (a shorter segment is its member's last)
-        tok_prefetch(w, G, g + gridDim.x, q, tn);
+        const uint32_t tokbase = sload_u32(w.seg_tokbase, g);
+        reinterpret_cast<uint4 *>(tokL)[tid] = pf.q;
+        rawL[tid] = pf.raw;
+        if (tid == 0 && L == SEG) tokL[SEG] = pf.tn;  // (a shorter segment is its member's last)
+        // the stores above wait for this segment's loads only if the next
+        // segment's are not yet issued: keep them apart
+        asm volatile("" ::: "memory");
+        tok_prefetch(w, G, g + gridDim.x, pf);
         for (uint32_t i = tid; i < NSYM; i += K3_T) hist[i] = 0;
         if (tid < 64) vb[tid] = 0;
         __syncthreads();
@@ -976,7 +1001,7 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
             if (!((V >> lane) & 1)) continue;
             const uint32_t x = bs + lane;
             uint32_t t = tokL[x];
-            if ((demb[b] >> lane) & 1) t = w.in[md.in_off + S + x];
+            if ((demb[b] >> lane) & 1) t = (rawL[x >> 2] >> (8 * (x & 3))) & 0xFFu;
             outp[cntb[b] + __popcll(V & lt)] = t;
             const uint32_t len = t >> 16, v = t & 0xFFFFu;
             if (len == 0) {
@@ -1021,179 +1046,137 @@ __global__ __launch_bounds__(1024) void k_hist_tokens(Work w) {
 // k_huffman: one wave per member
 
 struct HuffScratch {
-    __align__(16) uint32_t key[288];  // leaf (freq << 9 | id), 0xFFFFFFFF when unused
-    uint32_t sk[320];       // keys in sorted order
-    uint16_t par[2][576];   // node parent; leaf i -> node i, internal k -> node 288 + k
-    uint16_t dep[2][576];
+    __align__(16) uint32_t ck[292];  // used leaves' keys (freq << 10 | id), then 4 x ~0
+    uint32_t sl[288];                 // the same keys sorted
+    uint32_t ik[288];                 // internal nodes' keys (freq << 10 | 1 << 9 | index)
+    uint16_t hg[288];                 // internal nodes' heights
+    uint16_t par[576];                // parent: leaf id -> par[id], internal j -> par[288 + j]
+    uint16_t rb[290];                 // first internal node of each merge round
+    uint8_t dep[288];                 // internal nodes' depths
 };
-
-// Register j (uniform, 0..4) of a 5-register array: element t of the array
-// lives in lane t % 64 of register t / 64.
-__device__ __forceinline__ uint32_t rd5_blk(const uint32_t (&a)[5], uint32_t j) {
-    switch (j) {
-        case 0: return a[0];
-        case 1: return a[1];
-        case 2: return a[2];
-        case 3: return a[3];
-        default: return a[4];
-    }
-}
-__device__ __forceinline__ void wr5_blk(uint32_t (&a)[5], uint32_t j, uint32_t v) {
-    switch (j) {
-        case 0: a[0] = v; break;
-        case 1: a[1] = v; break;
-        case 2: a[2] = v; break;
-        case 3: a[3] = v; break;
-        default: a[4] = v; break;
-    }
-}
 
 // try_build_code_lengths (huffman.rs:26-60) for n <= 288 symbols on one
 // wave.  The reference pops the minimum (freq, id) from a binary heap whose
 // internal nodes get ids n, n+1, ...; merged frequencies never decrease and
-// internal ids exceed every leaf id, so merging the (freq, id)-sorted leaves
-// with a FIFO of internal nodes (leaf first on equal frequency) pops the same
-// sequence.  Leaves are ranked by all lanes; the merge state is wave-uniform
-// (scalar registers) with the sorted leaves and the internal-node FIFO held
-// one element per lane in registers; depths (= code lengths) come from
-// pointer jumping over the parent links.  Frequencies are halved while the
-// longest code exceeds `limit` (huffman.rs:44-58).
+// internal ids exceed every leaf id, so every pop takes the least item under
+// the order (freq, leaf before internal, leaf id / creation index).  Items
+// are 32-bit keys freq << 10 | internal << 9 | index in that order (totals
+// stay below 2^22: a histogram counts at most one token per input byte of a
+// 128 KiB member).
+//
+// Merge rounds: the 64 least remaining items are the least 64 of the 64
+// least leaves and the 64 least queued internal nodes, sorted by one bitonic
+// merge across the wave.  Let N = S0 + S1 be the next internal node.  Every
+// node made after it is larger, and an item is popped before N exactly when
+// its frequency is at most N's, so the heap pops S0, S1, S2, ... up to the
+// last such item before it ever takes a new node: those pairs are merged at
+// once, one per lane.  A round merges at least one pair; a node's parent is
+// made in a later round, so depths follow from the rounds in reverse.
+// Frequencies are halved while the longest code exceeds `limit`
+// (huffman.rs:44-58).
 __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens, HuffScratch &hs) {
     const uint32_t lane = threadIdx.x;
+    const uint64_t lt = (1ull << lane) - 1;
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
     uint32_t fr[5];
-    uint32_t used = 0;
 #pragma unroll
-    for (int j = 0; j < 5; j++) {
-        const uint32_t i = lane + 64 * j;
-        fr[j] = (int)i < n ? f[i] : 0u;
-        used += __popcll(__ballot(fr[j] != 0));
-    }
-    used = uni(used);
-    if (used <= 1) {
-#pragma unroll
-        for (int j = 0; j < 5; j++)
-            if ((int)(lane + 64 * j) < n) lens[lane + 64 * j] = fr[j] ? 1 : 0;
-        __syncthreads();
-        return;
-    }
-    const uint32_t root = 288 + used - 2;
+    for (int j = 0; j < 5; j++) fr[j] = (int)(lane + 64 * j) < n ? f[lane + 64 * j] : 0u;
     for (;;) {
+        // the used leaves' keys, compacted
+        uint32_t u = 0;
 #pragma unroll
         for (int j = 0; j < 5; j++) {
-            const uint32_t i = lane + 64 * j;
-            if ((int)i < n) hs.key[i] = fr[j] ? (fr[j] << 9) | i : 0xFFFFFFFFu;
-            else if (i < 288) hs.key[i] = 0xFFFFFFFFu;
+            const uint64_t m = __ballot(fr[j] != 0);
+            if (fr[j]) hs.ck[u + __popcll(m & lt)] = (fr[j] << 10) | (lane + 64 * j);
+            u += __popcll(m);
         }
+        u = uni(u);
+        if (u <= 1) {
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                if ((int)(lane + 64 * j) < n) lens[lane + 64 * j] = fr[j] ? 1 : 0;
+            __syncthreads();
+            return;
+        }
+        if (lane < 4) hs.ck[u + lane] = NONE;
         __syncthreads();
-        // rank of each used leaf among the used leaves by (freq, id)
+        // sorted by rank (keys are distinct)
         {
             uint32_t kk[5], r[5];
 #pragma unroll
             for (int j = 0; j < 5; j++) {
-                kk[j] = (int)(lane + 64 * j) < n ? hs.key[lane + 64 * j] : 0xFFFFFFFFu;
+                kk[j] = lane + 64 * j < u ? hs.ck[lane + 64 * j] : NONE;
                 r[j] = 0;
             }
-            for (int i = 0; i < n; i += 4) {
-                const uint4 k4 = *reinterpret_cast<const uint4 *>(&hs.key[i]);
+            for (uint32_t i = 0; i < u; i += 4) {
+                const uint4 k4 = *reinterpret_cast<const uint4 *>(&hs.ck[i]);
 #pragma unroll
                 for (int j = 0; j < 5; j++) r[j] += (k4.x < kk[j]) + (k4.y < kk[j]) + (k4.z < kk[j]) + (k4.w < kk[j]);
             }
 #pragma unroll
             for (int j = 0; j < 5; j++)
-                if (kk[j] != 0xFFFFFFFFu) hs.sk[r[j]] = kk[j];
+                if (kk[j] != NONE) hs.sl[r[j]] = kk[j];
         }
         __syncthreads();
-        uint32_t sk[5], iq[5];
+        // merge rounds
+        uint32_t nl = 0, qh = 0, k = 0, nr = 0;
+        while (k + 1 < u) {
+            // leaves ascending, queued internal nodes descending: one bitonic
+            // sequence of 128 whose lower half after the first exchange holds
+            // the 64 least; half-cleaners sort it
+            const uint32_t a = nl + lane < u ? hs.sl[nl + lane] : NONE;
+            const uint32_t bi = qh + 63 - lane;
+            const uint32_t b = bi < k ? hs.ik[bi] : NONE;
+            uint32_t v = min(a, b);
 #pragma unroll
-        for (int j = 0; j < 5; j++) {
-            const uint32_t t = lane + 64 * j;
-            sk[j] = t < used ? hs.sk[t] : 0xFFFFFFFFu;
-            iq[j] = 0;
-        }
-        // merge: wave-uniform control, one pick per step from the leaf list
-        // or the internal FIFO.  All three cursors only move forward, so each
-        // works on one 64-element block held in a register and changes block
-        // at most four times: a pick is one readlane.  Internal nodes carry
-        // freq << 9 | height, so the root's height -- the longest code --
-        // is known without a depth pass.
-        uint32_t root_h = 0;
-        {
-            const uint32_t me = lane;
-            uint32_t nl = 0, qh = 0, k = 0;
-            uint32_t lblk = sk[0];   // leaf block of nl
-            uint32_t wblk = 0;       // internal block being written (k)
-            uint32_t rblk = 0;       // internal block being read (qh), when it is not wblk
-            uint32_t lkey = __builtin_amdgcn_readlane(lblk, 0);
-            for (; k + 1 < used; k++) {
-                uint32_t fsum = 0, hgt = 0;
-#pragma unroll
-                for (int pick = 0; pick < 2; pick++) {
-                    const bool have_int = qh < k;
-                    uint32_t qv = 0;
-                    if (have_int) {
-                        const uint32_t src = ((qh ^ k) >> 6) ? rblk : wblk;
-                        qv = __builtin_amdgcn_readlane(src, qh & 63u);
-                    }
-                    if (nl < used && (!have_int || (lkey >> 9) <= (qv >> 9))) {
-                        fsum += lkey >> 9;
-                        if (me == 0) hs.par[0][lkey & 0x1FFu] = (uint16_t)(288 + k);
-                        nl++;
-                        if ((nl & 63u) == 0) lblk = rd5_blk(sk, nl >> 6);
-                        lkey = nl < used ? __builtin_amdgcn_readlane(lblk, nl & 63u) : 0xFFFFFFFFu;
-                    } else {
-                        fsum += qv >> 9;
-                        hgt = max(hgt, qv & 0x1FFu);
-                        if (me == 0) hs.par[0][288 + qh] = (uint16_t)(288 + k);
-                        qh++;
-                        // entering a block the writer has already left: fetch it
-                        if ((qh & 63u) == 0 && (qh >> 6) != (k >> 6)) rblk = rd5_blk(iq, qh >> 6);
-                    }
-                }
-                root_h = hgt + 1;
-                wblk = me == (k & 63u) ? (fsum << 9) | root_h : wblk;
-                if (((k + 1) & 63u) == 0) {
-                    // the writer leaves its block: keep it for the reader
-                    wr5_blk(iq, k >> 6, wblk);
-                    if ((qh >> 6) == (k >> 6)) rblk = wblk;
-                    wblk = 0;
-                }
+            for (int d = 32; d >= 1; d >>= 1) {
+                const uint32_t o = (uint32_t)__shfl_xor((int)v, d, 64);
+                v = (lane & d) ? max(v, o) : min(v, o);
             }
-            if (me == 0) hs.par[0][root] = (uint16_t)root;
-        }
-        if (root_h > (uint32_t)limit) {
-            // too long: halve every frequency > 1 and rebuild (huffman.rs:44-58)
+            const uint32_t thr = (uni(v) >> 10) + (__builtin_amdgcn_readlane(v, 1) >> 10);
+            const uint32_t T = __popcll(__ballot((v >> 10) <= thr)) >> 1;  // >= 1
+            const uint32_t x = (uint32_t)__shfl((int)v, (int)(2 * lane) & 63, 64);
+            const uint32_t y = (uint32_t)__shfl((int)v, (int)(2 * lane + 1) & 63, 64);
+            if (lane == 0) hs.rb[nr] = (uint16_t)k;
+            if (lane < T) {
+                const uint32_t node = k + lane;
+                const bool xi = (x >> 9) & 1u, yi = (y >> 9) & 1u;
+                const uint32_t hx = xi ? hs.hg[x & 0x1FFu] : 0u, hy = yi ? hs.hg[y & 0x1FFu] : 0u;
+                hs.ik[node] = (((x >> 10) + (y >> 10)) << 10) | 0x200u | node;
+                hs.hg[node] = (uint16_t)(max(hx, hy) + 1);
+                hs.par[(xi ? 288u : 0u) + (x & 0x1FFu)] = (uint16_t)node;
+                hs.par[(yi ? 288u : 0u) + (y & 0x1FFu)] = (uint16_t)node;
+            }
+            const uint32_t lu = __popcll(__ballot(lane < 2 * T && !((v >> 9) & 1u)));
+            nl += lu;
+            qh += 2 * T - lu;
+            k += T;
+            nr++;
             __syncthreads();
+        }
+        const uint32_t root = u - 2;
+        if (hs.hg[root] > (uint32_t)limit) {
+            // too long: halve every frequency > 1 and rebuild (huffman.rs:44-58)
 #pragma unroll
             for (int j = 0; j < 5; j++)
                 if (fr[j] > 1) fr[j] >>= 1;
+            __syncthreads();
             continue;
         }
-        __syncthreads();
-        // depth of every node (leaves 0..n-1, internal nodes 288 ..
-        // 288+used-2): pointer jumping; depths are below used, so
-        // ceil(log2(used)) rounds reach the root
-        const uint32_t nodes = (uint32_t)n + used - 1;  // leaf slots, then internal nodes
-        for (uint32_t t = lane; t < nodes; t += 64) {
-            const uint32_t id = t < (uint32_t)n ? t : 288 + (t - n);
-            if (id >= 288 || hs.key[id] != 0xFFFFFFFFu) hs.dep[0][id] = id == root ? 0 : 1;
+        // depths: the root's round last, then back to the first
+        if (lane == 0) {
+            hs.rb[nr] = (uint16_t)k;
+            hs.dep[root] = 0;
         }
         __syncthreads();
-        int cur = 0;
-        const int rounds = 32 - __builtin_clz(used - 1);
-        for (int r = 0; r < rounds; r++) {
-            for (uint32_t t = lane; t < nodes; t += 64) {
-                const uint32_t id = t < (uint32_t)n ? t : 288 + (t - n);
-                if (id < 288 && hs.key[id] == 0xFFFFFFFFu) continue;
-                const uint32_t pa = hs.par[cur][id];
-                hs.dep[cur ^ 1][id] = hs.dep[cur][id] + hs.dep[cur][pa];
-                hs.par[cur ^ 1][id] = hs.par[cur][pa];
-            }
+        for (int r = (int)nr - 2; r >= 0; r--) {
+            const uint32_t node = hs.rb[r] + lane;
+            if (node < hs.rb[r + 1]) hs.dep[node] = hs.dep[hs.par[288 + node]] + 1;
             __syncthreads();
-            cur ^= 1;
         }
 #pragma unroll
         for (int j = 0; j < 5; j++)
-            if ((int)(lane + 64 * j) < n) lens[lane + 64 * j] = fr[j] ? (uint8_t)hs.dep[cur][lane + 64 * j] : 0;
+            if ((int)(lane + 64 * j) < n) lens[lane + 64 * j] = fr[j] ? (uint8_t)(hs.dep[hs.par[lane + 64 * j]] + 1) : 0;
         __syncthreads();
         return;
     }
