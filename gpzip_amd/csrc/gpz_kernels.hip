@@ -244,6 +244,14 @@ __device__ __forceinline__ uint32_t ld4(const uint32_t *win, uint32_t x) {
 // set of positions it visits inside the block; across blocks the workgroup
 // follows those block exits (a segment has 64 blocks: at most 64 hops).
 
+// Orders one wave's LDS stores before its later LDS loads (wave-private
+// data: no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // Successor of segment position x.  next_known: tokL[L] holds the next
 // segment's first token.  absorb_last: the last position's successor depends
 // on the next segment and is left to k_chain (self-loop).
@@ -286,6 +294,30 @@ __device__ __forceinline__ uint32_t block_walk(uint32_t J, uint32_t bs, uint32_t
         }
     }
     mask = M;
+    return bs + j;
+}
+
+// The same doubling with the number of positions visited inside the block in
+// place of the set: the sets joined in a round are disjoint stretches of the
+// orbit (a self-loop adds nothing), so counts add.
+__device__ __forceinline__ uint32_t block_walk_count(uint32_t J, uint32_t bs, uint32_t L, bool valid, bool selfloop,
+                                                     uint32_t &cnt) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t lim = min(64u, L - bs);
+    uint32_t j = valid ? J - bs : 64u;
+    uint32_t c = (valid && !selfloop) ? 1u : 0u;
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        const bool inside = j < lim;
+        const int src = inside ? (int)j : (int)lane;
+        const uint32_t jj = (uint32_t)__shfl((int)j, src, 64);
+        const uint32_t cc = (uint32_t)__shfl((int)c, src, 64);
+        if (inside) {
+            j = jj;
+            c += cc;
+        }
+    }
+    cnt = c;
     return bs + j;
 }
 
@@ -428,10 +460,10 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
 // workgroups per CU.
 
 // The walk summary of segment g from its tokens in LDS (tokL[0..L)); 1024
-// threads.  Ea/Ca are two ping-pong pairs of 4096 u16.  Ends before the
-// caller's closing barrier.
+// threads.  P is a ping-pong pair of 4096 words exit | count << 16.  Ends
+// before the caller's closing barrier.
 __device__ __forceinline__ void seg_summary(const Work &w, uint32_t g, uint32_t L, bool last_seg, bool any_match,
-                                            const uint32_t *tokL, uint16_t (*Ea)[SEG], uint16_t (*Ca)[SEG]) {
+                                            const uint32_t *tokL, uint32_t *P0, uint32_t *P1) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const bool absorb_last = !w.strict && !last_seg;
     uint32_t *out = w.wsum + (uint64_t)g * SPAN;
@@ -451,33 +483,31 @@ __device__ __forceinline__ void seg_summary(const Work &w, uint32_t g, uint32_t 
         const bool valid = x < L;
         bool dem;
         const uint32_t J = valid ? succ_of(tokL, x, L, false, w.strict, absorb_last, dem) : 0;
-        uint64_t M;
-        const uint32_t ex = block_walk(J, bs, L, valid, valid && x == absorb, M);
-        if (valid) {
-            Ea[0][x] = (uint16_t)ex;
-            Ca[0][x] = (uint16_t)__popcll(M);
-        }
+        uint32_t c;
+        const uint32_t ex = block_walk_count(J, bs, L, valid, valid && x == absorb, c);
+        if (valid) P0[x] = ex | (c << 16);
     }
     __syncthreads();
     // level 2: pointer doubling over the block exits.  Every exit lies in a
     // later block, so 64 hops (6 rounds) reach the segment end or the
     // absorbing last position (its own exit, count 0).
-    uint32_t cur = 0;
+    uint32_t *cur = P0, *nxt = P1;
     for (int r = 0; r < 6; r++) {
         for (uint32_t x = tid; x < L; x += 1024) {
-            const uint32_t y = Ea[cur][x];
-            uint32_t e2 = y, c2 = Ca[cur][x];
+            const uint32_t v = cur[x], y = v & 0xFFFFu;
+            uint32_t nv = v;
             if (y < L) {
-                e2 = Ea[cur][y];
-                c2 += Ca[cur][y];
+                const uint32_t v2 = cur[y];
+                nv = (v2 & 0xFFFFu) | (((v >> 16) + (v2 >> 16)) << 16);
             }
-            Ea[cur ^ 1][x] = (uint16_t)e2;
-            Ca[cur ^ 1][x] = (uint16_t)c2;
+            nxt[x] = nv;
         }
         __syncthreads();
-        cur ^= 1;
+        uint32_t *t = cur;
+        cur = nxt;
+        nxt = t;
     }
-    for (uint32_t x = tid; x < L && x < SPAN; x += 1024) out[x] = Ea[cur][x] | ((uint32_t)Ca[cur][x] << 16);
+    for (uint32_t x = tid; x < L && x < SPAN; x += 1024) out[x] = cur[x];
 }
 
 constexpr int K1_T = 1024;
@@ -631,28 +661,41 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                     const uint2 bk = reinterpret_cast<const uint2 *>(T + TAB_BITS)[tid];
                     const uint32_t rkk = T[TAB_RANK + tid];
                     const uint32_t o0 = bits[2 * tid], o1 = bits[2 * tid + 1];
-                    uint32_t I0 = o0 & bk.x, I1 = o1 & bk.y;
-                    if (I0 | I1) {
+                    const uint32_t I0 = o0 & bk.x, I1 = o1 & bk.y;
+                    // the wave's common hashes as (id, entry index) pairs in its
+                    // queue, then one entry load per lane, 64 at a time
+                    const uint32_t c = __popc(I0) + __popc(I1);
+                    const uint32_t incl = wave_scan_incl(c);
+                    const uint32_t tot = __builtin_amdgcn_readlane(incl, 63);
+                    if (tot) {
+                        const uint32_t lane = tid & 63u;
+                        uint32_t *q = kq + (tid & ~63u);
                         const uint32_t ro0 = rnk[2 * tid], ro1 = rnk[2 * tid + 1];
                         const uint32_t tag = k << 24;
-                        // one common hash of each word per step (two independent loads)
-                        while (I0 | I1) {
-                            uint32_t i0 = 0, i1 = 0, q0 = EMPTY, q1 = EMPTY;
-                            if (I0) {
-                                const uint32_t m = (1u << __builtin_ctz(I0)) - 1u;
-                                q0 = ro0 + __popc(o0 & m);
-                                i0 = (rkk & 0xFFFFu) + __popc(bk.x & m);
-                                I0 &= I0 - 1u;
+                        for (uint32_t r0 = 0; r0 < tot; r0 += 64) {
+                            uint32_t J0 = I0, J1 = I1, idx = incl - c;
+                            while (J0 | J1) {
+                                uint32_t id, ent;
+                                if (J0) {
+                                    const uint32_t m = (1u << __builtin_ctz(J0)) - 1u;
+                                    id = ro0 + __popc(o0 & m);
+                                    ent = (rkk & 0xFFFFu) + __popc(bk.x & m);
+                                    J0 &= J0 - 1u;
+                                } else {
+                                    const uint32_t m = (1u << __builtin_ctz(J1)) - 1u;
+                                    id = ro1 + __popc(o1 & m);
+                                    ent = (rkk >> 16) + __popc(bk.y & m);
+                                    J1 &= J1 - 1u;
+                                }
+                                if (idx - r0 < 64u) q[idx - r0] = id | (ent << 12);
+                                idx++;
                             }
-                            if (I1) {
-                                const uint32_t m = (1u << __builtin_ctz(I1)) - 1u;
-                                q1 = ro1 + __popc(o1 & m);
-                                i1 = (rkk >> 16) + __popc(bk.y & m);
-                                I1 &= I1 - 1u;
+                            wave_lds_sync();
+                            if (r0 + lane < tot) {
+                                const uint32_t it = q[lane];
+                                fl[it & 0xFFFu] = tag | T[TAB_ENT + (it >> 12)];
                             }
-                            const uint32_t e0 = T[TAB_ENT + i0], e1 = T[TAB_ENT + i1];
-                            if (q0 != EMPTY) fl[q0] = tag | e0;
-                            if (q1 != EMPTY) fl[q1] = tag | e1;
+                            wave_lds_sync();
                         }
                     }
                 }
@@ -719,18 +762,14 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                         const uint32_t idx = bs[i] + __popcll(bm[i] & lt) - r0;
                         if (pd[i] && idx < 64) q[idx] = (x0 + i) | ((pd[i] & 0xFFFFu) << 12);
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    wave_lds_sync();
                     if (r0 + lane < T) {
                         const uint32_t it = q[lane];
                         const uint32_t x = it & 0xFFFu;
                         const uint32_t lim = min(n - (S + x), MAX_MATCH);
                         q[lane] = ext_len(win, S - WB + x, it >> 12, 16, lim);
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    wave_lds_sync();
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
                         const uint32_t idx = bs[i] + __popcll(bm[i] & lt) - r0;
@@ -741,9 +780,7 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                             if (((pd[i] >> 30) & 1u) || len > (st[i] >> 16)) st[i] = (len << 16) | d;
                         }
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    wave_lds_sync();
                 }
             }
             int active = 0;
@@ -769,8 +806,8 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
         }
         if (do_summary) {
             // the segment's walk summary from the same tokens, in the LDS the
-            // match search no longer needs: tokens over the window, the two
-            // exit arrays after them, the two count arrays over fl
+            // match search no longer needs: tokens over the window, one
+            // exit | count array after them, the other over fl
             __syncthreads();
             uint32_t *tokL = win;
             int anym = 0;
@@ -780,8 +817,7 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                 anym |= (x0 + i < L) && (t4[i] >> 16) != 0;
             }
             const int any_match = __syncthreads_or(anym);
-            seg_summary(w, g, L, E == n, any_match != 0, tokL, reinterpret_cast<uint16_t (*)[SEG]>(win + SEG),
-                        reinterpret_cast<uint16_t (*)[SEG]>(fl));
+            seg_summary(w, g, L, E == n, any_match != 0, tokL, win + SEG, fl);
         }
         __syncthreads();
     }
@@ -797,7 +833,7 @@ constexpr int KW_T = 1024;
 
 __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
     __shared__ uint32_t tokL[SEG];
-    __shared__ uint16_t Ea[2][SEG], Ca[2][SEG];
+    __shared__ uint32_t P[2][SEG];
     const uint32_t tid = threadIdx.x;
     const uint32_t G = uni(w.counts->nseg);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
@@ -811,7 +847,7 @@ __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
             anym |= (t >> 16) != 0;
         }
         const int any_match = __syncthreads_or(anym);
-        seg_summary(w, g, L, E == n, any_match != 0, tokL, Ea, Ca);
+        seg_summary(w, g, L, E == n, any_match != 0, tokL, P[0], P[1]);
         __syncthreads();
     }
 }
