@@ -233,9 +233,9 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
         p->grids.emit = p->grids.huff = p->grids.chain = std::max<uint32_t>(1, p->pl.M);
     }
     if (tune) {
-        if (tune->k1_variant > 2) {
+        if (tune->k1_variant > 2 || tune->emit_variant > 2) {
             delete p;
-            return fail(GPZ_E_INVAL, "k1_variant must be 0, 1 or 2");
+            return fail(GPZ_E_INVAL, "k1_variant and emit_variant must be 0, 1 or 2");
         }
         apply_tune(p->grids, *tune);
     } else if (flags & GPZ_PLAN_PERSISTENT) {
@@ -333,17 +333,20 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
             }
         }
     }
-    if (!rc) {  // bit emission
-        const uint32_t cs[] = {cap(cu, M), cap((uint64_t)cu * 2, M), cap((uint64_t)cu * 4, M), M};
+    if (!rc) {  // bit emission: workgroup size x persistent grid
+        const Cand cs[] = {{cap(cu, M), 0}, {cap((uint64_t)cu * 2, M), 0}, {cap((uint64_t)cu * 4, M), 0}, {M, 0},
+                           {cap((uint64_t)cu * 8, M), 1}, {cap((uint64_t)cu * 16, M), 1}, {M, 1}};
         float bt = 1e30f;
-        for (uint32_t c : cs) {
+        for (const Cand &c : cs) {
             Grids g = best;
-            g.emit = c;
+            g.emit = c.grid;
+            g.emitv = c.variant;
             float t;
             if ((rc = timed(g, 3, 4, &t))) break;
             if (t < bt) {
                 bt = t;
-                best.emit = c;
+                best.emit = g.emit;
+                best.emitv = g.emitv;
             }
         }
     }
@@ -357,6 +360,7 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
     out->k3_grid = best.k3;
     out->emit_grid = best.emit;
     out->k1_variant = best.k1v + 1;
+    out->emit_variant = best.emitv + 1;
     if (ms) {
         ms[0] = t1;
         ms[1] = t0;

@@ -1523,9 +1523,14 @@ __global__ __launch_bounds__(1024) void k_scan(Work w) {
 // ---------------------------------------------------------------------------
 // k_emit: one workgroup per member
 
-constexpr int K6_T = 1024;
-constexpr int BUF_WORDS = 6160;  // 4096 tokens x 48 bits + carry
+// Block size BT: 1024 threads (least latency per member) or 256 (eight
+// members per CU, for batches).  A tile is 4 tokens per thread.
+template <int BT>
+struct EmitShape {
+    static constexpr int BUF_WORDS = (4 * BT * 48) / 32 + 16;  // tile x 48 bits + carry
+};
 
+template <int BT>
 struct Emitter {
     uint32_t *buf;
     uint8_t *out;
@@ -1554,7 +1559,7 @@ struct Emitter {
     // before contributions: clear the words the tile touches, seed the carry
     __device__ __forceinline__ void begin(uint32_t tile_bits) {
         uint32_t nw = ((uint32_t)(cursor & 31) + tile_bits + 31) >> 5;
-        for (uint32_t i = threadIdx.x; i < nw + 1; i += K6_T) buf[i] = i == 0 ? carry : 0;
+        for (uint32_t i = threadIdx.x; i < nw + 1; i += BT) buf[i] = i == 0 ? carry : 0;
         __syncthreads();
     }
     // after contributions: store complete words, keep the partial one
@@ -1563,23 +1568,24 @@ struct Emitter {
         uint32_t off0 = (uint32_t)(cursor & 31);
         uint32_t full = (off0 + tile_bits) >> 5;
         uint64_t w0 = cursor >> 5;
-        for (uint32_t i = threadIdx.x; i < full; i += K6_T) store_word(w0 + i, buf[i]);
+        for (uint32_t i = threadIdx.x; i < full; i += BT) store_word(w0 + i, buf[i]);
         carry = ((off0 + tile_bits) & 31) ? buf[full] : 0;
         cursor += tile_bits;
         __syncthreads();
     }
 };
 
-__global__ __launch_bounds__(K6_T) void k_emit(Work w) {
+template <int BT>
+__global__ __launch_bounds__(BT) void k_emit(Work w) {
     __shared__ uint32_t ctab[NSYM];
-    __shared__ uint32_t buf[BUF_WORDS];
+    __shared__ uint32_t buf[EmitShape<BT>::BUF_WORDS];
     __shared__ uint32_t tmp[32];
     const uint32_t tid = threadIdx.x;
     const uint32_t M = uni(w.counts->nmem);
     for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
         const MemberDesc md = load_member(w.mem, m);
-        for (uint32_t i = tid; i < NSYM; i += K6_T) ctab[i] = w.codes[(uint64_t)m * NSYM + i];
-        Emitter em;
+        for (uint32_t i = tid; i < NSYM; i += BT) ctab[i] = w.codes[(uint64_t)m * NSYM + i];
+        Emitter<BT> em;
         em.buf = buf;
         em.out = w.out;
         em.A = w.mem_off[m];
@@ -1593,13 +1599,13 @@ __global__ __launch_bounds__(K6_T) void k_emit(Work w) {
             const uint32_t pbits = w.prefix_bits[m];
             em.begin(pbits);
             const uint32_t *pw = w.prefix + (uint64_t)m * PREFIX_WORDS;
-            for (uint32_t i = tid; i * 32 < pbits; i += K6_T) em.or_bits(i * 32, pw[i], min(32u, pbits - i * 32));
+            for (uint32_t i = tid; i * 32 < pbits; i += BT) em.or_bits(i * 32, pw[i], min(32u, pbits - i * 32));
             em.end(pbits);
         }
         // tokens, 4096 per tile
         const uint32_t nt = w.ntok[m];
         const uint32_t *tk = w.walked + md.pos_base;
-        for (uint32_t t0 = 0; t0 < nt; t0 += 4 * K6_T) {
+        for (uint32_t t0 = 0; t0 < nt; t0 += 4 * BT) {
             uint32_t v[4][4], nb[4][4];
             uint32_t mybits = 0;
 #pragma unroll
@@ -1633,7 +1639,7 @@ __global__ __launch_bounds__(K6_T) void k_emit(Work w) {
                 mybits += nb[k][0] + nb[k][1] + nb[k][2] + nb[k][3];
             }
             uint32_t tile_bits;
-            uint32_t rel = block_scan<K6_T>(mybits, tmp, &tile_bits);
+            uint32_t rel = block_scan<BT>(mybits, tmp, &tile_bits);
             em.begin(tile_bits);
 #pragma unroll
             for (int k = 0; k < 4; k++)
@@ -1735,8 +1741,11 @@ void launch_huffman(const Work &w, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_huffman, dim3(grid), dim3(64), 0, st, w);
 }
 void launch_scan(const Work &w, hipStream_t st) { hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, w); }
-void launch_emit(const Work &w, uint32_t grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_emit, dim3(grid), dim3(K6_T), 0, st, w);
+void launch_emit(const Work &w, uint32_t grid, int variant, hipStream_t st) {
+    if (variant == 1)
+        hipLaunchKernelGGL(k_emit<256>, dim3(grid), dim3(256), 0, st, w);
+    else
+        hipLaunchKernelGGL(k_emit<1024>, dim3(grid), dim3(1024), 0, st, w);
 }
 
 }  // namespace gpz

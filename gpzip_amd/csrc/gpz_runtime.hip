@@ -36,7 +36,7 @@ void launch_walk_emit(const Work &w, uint32_t grid, hipStream_t st);
 void launch_hist_tokens(const Work &w, uint32_t grid, hipStream_t st);
 void launch_huffman(const Work &w, uint32_t grid, hipStream_t st);
 void launch_scan(const Work &w, hipStream_t st);
-void launch_emit(const Work &w, uint32_t grid, hipStream_t st);
+void launch_emit(const Work &w, uint32_t grid, int variant, hipStream_t st);
 
 static thread_local char g_err[512];
 
@@ -265,6 +265,7 @@ void apply_tune(Grids &g, const gpz_tune &t) {
     if (t.k3_grid) g.k3 = t.k3_grid;
     if (t.emit_grid) g.emit = t.emit_grid;
     if (t.k1_variant == 1 || t.k1_variant == 2) g.k1v = t.k1_variant - 1;
+    if (t.emit_variant == 1 || t.emit_variant == 2) g.emitv = t.emit_variant - 1;
 }
 
 // Enqueue the kernels of one run.  Events (if given) bracket the match-find
@@ -286,7 +287,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
             launch_huffman(w, g.huff, st);
             launch_scan(w, st);
             if (ev) hipEventRecord(ev[3], st);
-            launch_emit(w, g.emit, st);
+            launch_emit(w, g.emit, g.emitv, st);
             if (ev) hipEventRecord(ev[4], st);
             break;
         default:  // MODE_FULL
@@ -299,7 +300,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
             launch_huffman(w, g.huff, st);
             launch_scan(w, st);
             if (ev) hipEventRecord(ev[3], st);
-            launch_emit(w, g.emit, st);
+            launch_emit(w, g.emit, g.emitv, st);
             if (ev) hipEventRecord(ev[4], st);
             break;
     }
@@ -424,7 +425,8 @@ extern "C" {
 int gpz_set_tune(gpz_ctx *ctx, const gpz_tune *tune) {
     clear_error();
     if (!ctx) return fail(GPZ_E_INVAL, "null context");
-    if (tune && tune->k1_variant > 2) return fail(GPZ_E_INVAL, "k1_variant must be 0, 1 or 2");
+    if (tune && (tune->k1_variant > 2 || tune->emit_variant > 2))
+        return fail(GPZ_E_INVAL, "k1_variant and emit_variant must be 0, 1 or 2");
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (tune) ctx->tune = *tune;
     else ctx->tune = gpz_tune{};

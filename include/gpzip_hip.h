@@ -153,7 +153,9 @@ typedef struct gpz_tune {
     uint32_t emit_grid;  /* persistent grid of the bit-emission kernel */
     uint32_t k1_variant; /* match-find occupancy: 1 = two workgroups per CU (64 VGPRs),
                             2 = one workgroup per CU (128 VGPRs); 0 = built-in */
-    uint32_t reserved[3];
+    uint32_t emit_variant; /* bit-emission workgroup: 1 = 1024 threads per member,
+                              2 = 256 threads (eight members per CU); 0 = built-in (1) */
+    uint32_t reserved[2];
 } gpz_tune;
 
 /* Tuning record used by every batched launch of this context (gpz_batch_*,

@@ -210,13 +210,14 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     say("autotune")
     probe = torch.zeros(4 * o + (1 << 20), dtype=torch.uint8, device="cuda:0")
     tune, t_ms, t_def = ctx.autotune(descs, d_in.data_ptr(), probe.data_ptr())
-    assert tune.k1_variant in (1, 2) and tune.k1_grid >= 1 and tune.k0_grid >= 1
+    assert tune.k1_variant in (1, 2) and tune.emit_variant in (1, 2) and tune.k1_grid >= 1 and tune.k0_grid >= 1
     assert tune.k3_grid >= 1 and tune.emit_grid >= 1 and t_ms > 0 and t_def > 0
     assert run(tune) == ref
     for v in (1, 2):
         for g in (1, 7, 256, 100000):
             t = gpzip_amd.Tune()
             t.k0_grid, t.k1_grid, t.k3_grid, t.emit_grid, t.k1_variant = g, g, g, max(1, g // 3), v
+            t.emit_variant = 3 - v
             say("shape", v, g)
             assert run(t) == ref
     # the context-wide record drives the batch API (graph re-captured on change)
@@ -229,10 +230,11 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     finally:
         ctx.set_tune(None)
     assert ctx.get_tune().k1_grid == 0
-    bad = gpzip_amd.Tune()
-    bad.k1_variant = 3
-    with pytest.raises(gpzip_amd.GpzError):
-        ctx.set_tune(bad)
+    for field in ("k1_variant", "emit_variant"):
+        bad = gpzip_amd.Tune()
+        setattr(bad, field, 3)
+        with pytest.raises(gpzip_amd.GpzError):
+            ctx.set_tune(bad)
 
 
 def test_server_alone_equals_mix(ctx):
