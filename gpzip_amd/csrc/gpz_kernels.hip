@@ -352,28 +352,34 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
     const uint32_t tid = threadIdx.x;
     const uint32_t G = uni(w.counts->nseg);
     for (uint32_t i = tid; i < 256; i += K0_T) crc_tab[i] = c_crc_tab[i];
+    // bytes x0 .. x0+11 of a segment (buffers are padded past every member),
+    // loaded one segment ahead so the loads are in flight while the current
+    // one is processed
+    const uint32_t x0 = tid * 8;
+    auto fetch = [&](uint32_t gg, uint32_t &a, uint32_t &b, uint32_t &c) {
+        a = b = c = 0;
+        if (gg >= G) return;
+        const MemberDesc mm = load_member(w.mem, sload_u32(w.seg_mem, gg));
+        const uint32_t SS = (gg - mm.seg_base) << SEG_LOG2, LL = min(SS + SEG, mm.n) - SS;
+        if (x0 < LL) {
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(w.in + mm.in_off + SS + x0);
+            a = src[0];
+            b = src[1];
+            c = src[2];
+        }
+    };
+    const uint32_t fold = c_crc_fold[tid];  // x^(8 * bytes after this thread's 8) for a full segment
+    uint32_t n0, n1, n2;
+    fetch(xcd_block(), n0, n1, n2);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
-        const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
+        const MemberDesc md = load_member(w.mem, sload_u32(w.seg_mem, g));
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
         uint32_t *T = w.segtab + (uint64_t)g * TAB_WORDS;
-        // bytes x0 .. x0+11 of the segment (buffers are padded past every member)
-        const uint32_t x0 = tid * 8;
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(w.in + md.in_off + S + x0);
-        uint32_t d0 = 0, d1 = 0, d2 = 0;
-        if (x0 < L) {
-            d0 = src[0];
-            d1 = src[1];
-            d2 = src[2];
-        }
+        const uint32_t d0 = n0, d1 = n1, d2 = n2;
         const uint64_t lo = (uint64_t)d0 | ((uint64_t)d1 << 32);
 #pragma unroll
         for (int r = 0; r < 4; r++) bm[tid + r * K0_T] = 0;
-#pragma unroll
-        for (int r = 0; r < (int)(SEG / K0_T); r++) {
-            fst[tid + r * K0_T] = EMPTY;
-            lst[tid + r * K0_T] = 0;
-        }
         __syncthreads();
         uint32_t hh[8];
         uint32_t hvm = 0;
@@ -388,6 +394,10 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
             hvm |= (uint32_t)hv << i;
             if (hv) atomicOr(&bm[hh[i] >> 5], 1u << (hh[i] & 31));
         }
+        // the bytes are in registers now: the next segment's loads go out
+        // without a wait for them
+        asm volatile("" ::: "memory");
+        fetch(g + gridDim.x, n0, n1, n2);
         __syncthreads();
         uint32_t D;  // distinct hashes in the segment
         {
@@ -403,6 +413,10 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
             reinterpret_cast<uint4 *>(T + TAB_BITS)[tid] = q;
             reinterpret_cast<uint2 *>(T + TAB_RANK)[tid] = make_uint2(r0 | (r1 << 16), r2 | (r3 << 16));
             if (tid == 0) w.seg_ntab[g] = D;
+            for (uint32_t id = tid; id < D; id += K0_T) {
+                fst[id] = EMPTY;
+                lst[id] = 0;
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -428,7 +442,7 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
             }
             if (x0 < L) {
                 const uint32_t after = L - min(x0 + 8, L);
-                c = gf2_mul(c, L == SEG ? c_crc_fold[tid] : xpow8(after));
+                c = gf2_mul(c, L == SEG ? fold : xpow8(after));
             } else {
                 c = 0;
             }
