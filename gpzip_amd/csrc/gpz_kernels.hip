@@ -870,7 +870,7 @@ __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
 // k_chain: one wave per member
 
 __global__ __launch_bounds__(64) void k_chain(Work w) {
-    __shared__ uint32_t sum[32 * SPAN];
+    __shared__ __align__(16) uint32_t sum[32 * SPAN];
     __shared__ uint32_t btok[64];
     const uint32_t M = uni(w.counts->nmem);
     const uint32_t lane = threadIdx.x;
@@ -879,8 +879,20 @@ __global__ __launch_bounds__(64) void k_chain(Work w) {
         const uint32_t nseg = md.nseg;
         const bool cached = nseg <= 32;
         if (cached) {
-            const uint32_t *src = w.wsum + (uint64_t)md.seg_base * SPAN;
-            for (uint32_t i = lane; i < nseg * SPAN; i += 64) sum[i] = src[i];
+            // the member's summaries (8-byte aligned: SPAN is even), eight
+            // loads in flight per lane before the first wait
+            const uint2 *src = reinterpret_cast<const uint2 *>(w.wsum + (uint64_t)md.seg_base * SPAN);
+            uint2 *dst = reinterpret_cast<uint2 *>(sum);
+            const uint32_t n2 = nseg * (SPAN / 2);
+            for (uint32_t i0 = lane; i0 < n2; i0 += 64 * 8) {
+                uint2 v[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) v[j] = src[min(i0 + 64 * j, n2 - 1)];
+                asm volatile("" ::: "memory");
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (i0 + 64 * j < n2) dst[i0 + 64 * j] = v[j];
+            }
             if (lane + 1 < nseg) {
                 uint64_t e = md.pos_base + ((uint64_t)(lane + 1) << SEG_LOG2);
                 btok[2 * lane] = w.tok[e - 1];
