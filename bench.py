@@ -6,7 +6,9 @@ compressed by the plain per-chunk form (one pipeline launch sequence per chunk,
 all chunks submitted on their own streams), inputs resident in HBM.  A step is
 one pass over the whole corpus.  Beside it: the batched form (one graph replay
 over all 128 chunks) on the same data, the roofline of the dominant kernel
-(match-find) from HIP events on its stream, and the CPU baselines.
+(match-find) from HIP events on its stream, the end-to-end rates through the
+drop-in boundary with host buffers (PCIe both ways; never the headline), and
+the CPU baselines.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mixed] [--serve]
 
@@ -172,6 +174,7 @@ def plain_and_batched(torch, ctx, data: np.ndarray, sizes, steps, warmup, check_
             assert chunks_out[i] == O.gzip_chunk(data[src:src + s]), f"chunk {i} differs from the oracle"
             src += s
     res["ratio"] = total_out / n_in
+    res["chunks_out"] = chunks_out
     res["n_in"] = n_in
     res["n_out"] = total_out
     res["members"] = int(sum((int(s) + 131071) // 131072 if s > 131072 else 1 for s in sizes))
@@ -222,10 +225,66 @@ def cpu_baselines(data: np.ndarray, sample_bytes: int, chunk: int):
     t0 = time.perf_counter()
     ztot, _ = O.zlib_chunks(data, chunk, 5, threads)
     dz = time.perf_counter() - t0
+    one = data[:16 * MiB]
+    t0 = time.perf_counter()
+    O.zlib_chunks(one, chunk, 5, 1)
+    d1 = time.perf_counter() - t0
     zl = {"value": data.size / dz / 1e9, "unit": "GB/s", "cores": threads,
           "kind": "gpzip-codec-cpu structure on system zlib " + O.zlib_version() + " level 5 (not zlib-rs)",
-          "ratio": ztot / data.size}
+          "ratio": ztot / data.size, "single_core_gbps": one.size / d1 / 1e9}
     return port, zl
+
+
+def end_to_end(ctx, data: np.ndarray, sizes, expect, threads: int = 16, reps: int = 3):
+    """Host buffers through the drop-in boundary: gpz_compress_chunk called
+    from `threads` workers at once (the reference's rayon ChunkFn callers), and
+    the whole list as one gpz_batch_submit.  PCIe both ways; never `value`."""
+    pieces, src = [], 0
+    for s in sizes:
+        pieces.append(np.ascontiguousarray(data[src:src + s]))
+        src += s
+    n_in = sum(sizes)
+    with cf.ThreadPoolExecutor(threads) as ex:
+        out = list(ex.map(ctx.compress_chunk, pieces))  # warm-up, and the check
+        assert out == expect, "gpz_compress_chunk differs from the device-resident plan"
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            list(ex.map(ctx.compress_chunk, pieces))
+        dp = (time.perf_counter() - t0) / reps
+    # the batch API on one contiguous host buffer (as a service holds it);
+    # results are pointers into the batch, read after the timed region
+    L = ctx.L
+    base = np.concatenate(pieces)
+    descs = (gpzip_amd.Desc * len(pieces))()
+    o = 0
+    for i, s in enumerate(sizes):
+        descs[i].offset, descs[i].len = o, s
+        o += s
+
+    def batch_once():
+        b = C.c_void_p()
+        gpzip_amd._check(L.gpz_batch_submit(ctx.h, C.c_void_p(base.ctypes.data), descs, len(pieces), C.byref(b)))
+        gpzip_amd._check(L.gpz_batch_wait(b))
+        return b
+
+    b = batch_once()
+    try:
+        got = []
+        for i in range(len(pieces)):
+            ptr, ln = C.POINTER(C.c_uint8)(), C.c_size_t()
+            gpzip_amd._check(L.gpz_batch_result(b, i, C.byref(ptr), C.byref(ln)))
+            got.append(C.string_at(ptr, ln.value))
+    finally:
+        L.gpz_batch_free(b)
+    assert got == expect, "gpz_batch_submit differs from the device-resident plan"
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.gpz_batch_free(batch_once())
+    db = (time.perf_counter() - t0) / reps
+    return {"plain_gbps": round(n_in / dp / 1e9, 3), "batched_gbps": round(n_in / db / 1e9, 3), "unit": "GB/s",
+            "plain_api": f"gpz_compress_chunk from {threads} threads, 2 MiB chunks, host buffers (PCIe both ways)",
+            "batched_api": "gpz_batch_submit + gpz_batch_wait over the 128 chunks in one host buffer, "
+                           "results in host memory (PCIe both ways)"}
 
 
 def profiled_traffic(n_in: int):
@@ -381,6 +440,8 @@ def main():
                      "emit_ms": round(r["emit_ms"], 4),
                      "pipeline_ms": round(r["pipeline_ms"], 4)},
     }
+    log("end to end (host buffers) ...")
+    line["end_to_end"] = end_to_end(ctx, data, sizes, r["chunks_out"])
     if not args.no_cpu:
         log("cpu baselines ...")
         port, zl = cpu_baselines(data, 64 * MiB, 2 * MiB)

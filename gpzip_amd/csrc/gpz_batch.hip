@@ -10,6 +10,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "gpz_internal.h"
@@ -25,6 +27,50 @@ using namespace gpz;
     } while (0)
 
 namespace gpz {
+
+// Host copies of a batch: one memcpy per chunk, split over up to 8 threads
+// (contiguous runs of chunks of about equal bytes) once the batch is large
+// enough for a single core's copy rate to matter.
+static void copy_pieces(uint8_t *dst, const uint64_t *dst_off, const uint8_t *const *src, const uint64_t *len,
+                        size_t n) {
+    uint64_t total = 0;
+    for (size_t i = 0; i < n; i++) total += len[i];
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t T = total < (16ull << 20) ? 1 : std::min<size_t>({8, hw, n});
+    auto run = [&](size_t a, size_t b) {
+        for (size_t i = a; i < b; i++)
+            if (len[i]) memcpy(dst + dst_off[i], src[i], len[i]);
+    };
+    if (T <= 1) {
+        run(0, n);
+        return;
+    }
+    std::vector<size_t> cut{0};
+    uint64_t acc = 0;
+    for (size_t i = 0; i < n && cut.size() < T; i++) {
+        acc += len[i];
+        if (acc * T >= total * cut.size()) cut.push_back(i + 1);
+    }
+    if (cut.back() != n) cut.push_back(n);
+    std::vector<std::thread> th;
+    for (size_t k = 1; k + 1 < cut.size(); k++) th.emplace_back(run, cut[k], cut[k + 1]);
+    run(cut[0], cut[1]);
+    for (auto &t : th) t.join();
+}
+
+// One large buffer copied by up to 8 threads.
+static void copy_bytes(uint8_t *dst, const uint8_t *src, uint64_t n) {
+    constexpr uint64_t PIECE = 4ull << 20;
+    const uint64_t np = (n + PIECE - 1) / PIECE;
+    std::vector<uint64_t> off(np), len(np);
+    std::vector<const uint8_t *> ptr(np);
+    for (uint64_t i = 0; i < np; i++) {
+        off[i] = i * PIECE;
+        len[i] = std::min(PIECE, n - off[i]);
+        ptr[i] = src + off[i];
+    }
+    copy_pieces(dst, off.data(), ptr.data(), len.data(), np);
+}
 
 // Capture the FULL-mode kernel sequence for w on st into *exec.
 static int capture_graph(const Work &w, const Grids &g, hipStream_t st, hipGraphExec_t *exec) {
@@ -52,9 +98,7 @@ int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *con
     if ((rc = ln->din.ensure(pl.in_bytes))) return rc;
     if ((rc = ln->hin.ensure(pl.in_bytes))) return rc;
     HIPCHK(hipSetDevice(ctx_device(ctx)));
-    uint8_t *h = (uint8_t *)ln->hin.p;
-    for (size_t c = 0; c < pl.C; c++)
-        if (lens[c]) memcpy(h + pl.chunk_dev_off[c], ptrs[c], lens[c]);
+    copy_pieces((uint8_t *)ln->hin.p, pl.chunk_dev_off.data(), ptrs, lens, pl.C);
     HIPCHK(hipMemcpyAsync(ln->din.p, ln->hin.p, pl.in_end, hipMemcpyHostToDevice, ln->st));
     if ((rc = ln->wsp.upload_plan(pl, ln->st))) return rc;
     Work w = ln->wsp.work;
@@ -93,14 +137,15 @@ int lane_batch_collect_pinned(Lane *ln, const Plan &pl, const uint8_t **data, st
     return GPZ_OK;
 }
 
-int lane_batch_collect(Lane *ln, const Plan &pl, std::vector<uint8_t> &out, std::vector<uint64_t> &off,
+int lane_batch_collect(Lane *ln, const Plan &pl, std::unique_ptr<uint8_t[]> &out, std::vector<uint64_t> &off,
                        std::vector<uint64_t> &len) {
     const uint8_t *d = nullptr;
     int rc = lane_batch_collect_pinned(ln, pl, &d, off, len);
     if (rc) return rc;
     uint64_t total = 0;
     for (size_t c = 0; c < pl.C; c++) total = std::max<uint64_t>(total, off[c] + len[c]);
-    out.assign(d, d + total);
+    out.reset(new uint8_t[std::max<uint64_t>(total, 1)]);  // no zero fill: every byte is copied
+    copy_bytes(out.get(), d, total);
     return GPZ_OK;
 }
 
@@ -110,7 +155,7 @@ struct gpz_batch {
     gpz_ctx *ctx = nullptr;
     Lane *lane = nullptr;
     Plan pl;
-    std::vector<uint8_t> out;
+    std::unique_ptr<uint8_t[]> out;
     std::vector<uint64_t> off, len;
     int status = GPZ_OK;
     bool done = false;
@@ -170,7 +215,7 @@ int gpz_batch_result(gpz_batch *b, size_t i, const uint8_t **data, size_t *len) 
     if (!b->done) return fail(GPZ_E_INVAL, "batch not waited");
     if (b->status) return b->status;
     if (i >= b->pl.C) return fail(GPZ_E_INVAL, "chunk index %zu out of range", i);
-    *data = b->out.data() + b->off[i];
+    *data = b->out.get() + b->off[i];
     *len = b->len[i];
     return GPZ_OK;
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <vector>
 
 #include "gpz_internal.h"
@@ -119,7 +120,7 @@ int lane_fetch_chunk(Lane *ln, uint32_t c, uint8_t *out, size_t out_cap, size_t 
 // the lane's graph.  Returns once enqueued.
 int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *const *ptrs, const uint64_t *lens);
 // Wait for the step and copy every chunk's members to host.
-int lane_batch_collect(Lane *ln, const Plan &pl, std::vector<uint8_t> &out, std::vector<uint64_t> &off,
+int lane_batch_collect(Lane *ln, const Plan &pl, std::unique_ptr<uint8_t[]> &out, std::vector<uint64_t> &off,
                        std::vector<uint64_t> &len);
 // Same, leaving the members in the lane's pinned staging (valid until the lane's next step).
 int lane_batch_collect_pinned(Lane *ln, const Plan &pl, const uint8_t **data, std::vector<uint64_t> &off,
