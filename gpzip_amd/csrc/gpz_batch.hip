@@ -28,15 +28,16 @@ using namespace gpz;
 
 namespace gpz {
 
-// Host copies of a batch: one memcpy per chunk, split over up to 8 threads
-// (contiguous runs of chunks of about equal bytes) once the batch is large
-// enough for a single core's copy rate to matter.
+// Host copies of a batch: one memcpy per chunk, split over up to max_threads
+// threads (contiguous runs of chunks of about equal bytes) once the batch is
+// large enough for a single core's copy rate to matter.  Callers that already
+// copy from several threads at once (the server's workers) pass 1.
 static void copy_pieces(uint8_t *dst, const uint64_t *dst_off, const uint8_t *const *src, const uint64_t *len,
-                        size_t n) {
+                        size_t n, unsigned max_threads) {
     uint64_t total = 0;
     for (size_t i = 0; i < n; i++) total += len[i];
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const size_t T = total < (16ull << 20) ? 1 : std::min<size_t>({8, hw, n});
+    const size_t T = total < (16ull << 20) ? 1 : std::min<size_t>({(size_t)max_threads, (size_t)hw, n});
     auto run = [&](size_t a, size_t b) {
         for (size_t i = a; i < b; i++)
             if (len[i]) memcpy(dst + dst_off[i], src[i], len[i]);
@@ -69,7 +70,7 @@ static void copy_bytes(uint8_t *dst, const uint8_t *src, uint64_t n) {
         len[i] = std::min(PIECE, n - off[i]);
         ptr[i] = src + off[i];
     }
-    copy_pieces(dst, off.data(), ptr.data(), len.data(), np);
+    copy_pieces(dst, off.data(), ptr.data(), len.data(), np, 8);
 }
 
 // Capture the FULL-mode kernel sequence for w on st into *exec.
@@ -92,13 +93,14 @@ static int capture_graph(const Work &w, const Grids &g, hipStream_t st, hipGraph
     return GPZ_OK;
 }
 
-int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *const *ptrs, const uint64_t *lens) {
+int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *const *ptrs, const uint64_t *lens,
+                      unsigned copy_threads) {
     int rc;
     if ((rc = ln->wsp.reserve(Caps{pl.P, pl.G, pl.M, pl.C, pl.out_cap}))) return rc;
     if ((rc = ln->din.ensure(pl.in_bytes))) return rc;
     if ((rc = ln->hin.ensure(pl.in_bytes))) return rc;
     HIPCHK(hipSetDevice(ctx_device(ctx)));
-    copy_pieces((uint8_t *)ln->hin.p, pl.chunk_dev_off.data(), ptrs, lens, pl.C);
+    copy_pieces((uint8_t *)ln->hin.p, pl.chunk_dev_off.data(), ptrs, lens, pl.C, copy_threads);
     HIPCHK(hipMemcpyAsync(ln->din.p, ln->hin.p, pl.in_end, hipMemcpyHostToDevice, ln->st));
     if ((rc = ln->wsp.upload_plan(pl, ln->st))) return rc;
     Work w = ln->wsp.work;
@@ -106,7 +108,7 @@ int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *con
     w.gzip = 1;
     w.strict = 0;
     Grids g = grids_for(ln->wsp.caps, ctx_cus(ctx));
-    apply_tune(g, ctx_tune(ctx));
+    apply_tune(g, ctx_tune(ctx), ln->wsp.caps.G, ln->wsp.caps.M);  // capacity: stable, so the graph is kept
     if (!ln->graph || ln->graph_gen != ln->wsp.generation || ln->graph_in != ln->din.p || !(ln->graph_grids == g)) {
         if ((rc = capture_graph(w, g, ln->st, &ln->graph))) return rc;
         ln->graph_gen = ln->wsp.generation;
@@ -185,7 +187,7 @@ int gpz_batch_submit(gpz_ctx *ctx, const uint8_t *base, const gpz_desc *descs, s
     b->lane = ln;
     std::vector<const uint8_t *> ptrs(n);
     for (size_t c = 0; c < n; c++) ptrs[c] = base + descs[c].offset;
-    rc = lane_batch_launch(ctx, ln, b->pl, ptrs.data(), lens.data());
+    rc = lane_batch_launch(ctx, ln, b->pl, ptrs.data(), lens.data(), 8);
     if (rc) {
         (void)hipStreamSynchronize(ln->st);
         ctx_release(ctx, ln);
@@ -282,9 +284,9 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
             delete p;
             return fail(GPZ_E_INVAL, "k1_variant and emit_variant must be 0, 1 or 2");
         }
-        apply_tune(p->grids, *tune);
+        apply_tune(p->grids, *tune, p->pl.G, p->pl.M);
     } else if (flags & GPZ_PLAN_PERSISTENT) {
-        apply_tune(p->grids, ctx_tune(ctx));
+        apply_tune(p->grids, ctx_tune(ctx), p->pl.G, p->pl.M);
     }
     *out = p;
     return GPZ_OK;

@@ -57,8 +57,9 @@ struct Grids {
 };
 
 Grids grids_for(const Caps &c, int n_cu);
-// Override the persistent grids with the non-zero fields of a tuning record.
-void apply_tune(Grids &g, const gpz_tune &t);
+// Override the persistent grids with the non-zero fields of a tuning record,
+// then cap the segment grids at G and the member grid at M.
+void apply_tune(Grids &g, const gpz_tune &t, uint64_t G, uint64_t M);
 void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t *ev);
 
 }  // namespace gpz
@@ -118,7 +119,8 @@ int lane_fetch_chunk(Lane *ln, uint32_t c, uint8_t *out, size_t out_cap, size_t 
 
 // Batched step on a lane: pack the chunks into pinned staging, upload, replay
 // the lane's graph.  Returns once enqueued.
-int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *const *ptrs, const uint64_t *lens);
+int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *const *ptrs, const uint64_t *lens,
+                      unsigned copy_threads);
 // Wait for the step and copy every chunk's members to host.
 int lane_batch_collect(Lane *ln, const Plan &pl, std::unique_ptr<uint8_t[]> &out, std::vector<uint64_t> &off,
                        std::vector<uint64_t> &len);

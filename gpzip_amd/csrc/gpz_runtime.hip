@@ -259,13 +259,20 @@ Grids grids_for(const Caps &c, int n_cu) {
     return g;
 }
 
-void apply_tune(Grids &g, const gpz_tune &t) {
+void apply_tune(Grids &g, const gpz_tune &t, uint64_t G, uint64_t M) {
     if (t.k0_grid) g.k0 = t.k0_grid;
     if (t.k1_grid) g.k1 = t.k1_grid;
     if (t.k3_grid) g.k3 = t.k3_grid;
     if (t.emit_grid) g.emit = t.emit_grid;
     if (t.k1_variant == 1 || t.k1_variant == 2) g.k1v = t.k1_variant - 1;
     if (t.emit_variant == 1 || t.emit_variant == 2) g.emitv = t.emit_variant - 1;
+    // a record tuned on a larger batch must not launch workgroups with no
+    // segment or member to take
+    auto cap = [](uint32_t v, uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(v, n)); };
+    g.k0 = cap(g.k0, G);
+    g.k1 = cap(g.k1, G);
+    g.k3 = cap(g.k3, G);
+    g.emit = cap(g.emit, M);
 }
 
 // Enqueue the kernels of one run.  Events (if given) bracket the match-find

@@ -101,7 +101,7 @@ struct gpz_server {
             }
             Plan pl;
             int rc = build_plan(pl, lens.data(), nullptr, step.size(), ctx_sub_chunk(ctx), 1);
-            if (!rc) rc = lane_batch_launch(ctx, ln, pl, ptrs.data(), lens.data());
+            if (!rc) rc = lane_batch_launch(ctx, ln, pl, ptrs.data(), lens.data(), 1);  // workers copy in parallel already
             const uint8_t *res = nullptr;
             if (!rc) rc = lane_batch_collect_pinned(ln, pl, &res, off, len);
             if (rc) (void)hipStreamSynchronize(ln->st);
