@@ -355,12 +355,17 @@ def serve(torch, ctx, n_requests, load_gbps, seed=4, **server_cfg):
             time.sleep(next_t - now)
         src = pools[int(choose[i] * 3)]
         srv.submit(src[offs[i]:offs[i] + sizes[i]], i)
+    t_arrived = time.perf_counter()
     th.join()
     dt = time.perf_counter() - t0
     stats = srv.stats()
     srv.stop()
     lat = np.array(lat)
-    return {"requests": n_requests, "offered_gbps": load_gbps, "sustained_gbps": float(sizes.sum() / dt / 1e9),
+    # arrived_gbps: the rate the submit loop actually reached; below offered_gbps when
+    # submission itself fell behind the schedule
+    return {"requests": n_requests, "offered_gbps": load_gbps,
+            "arrived_gbps": float(sizes.sum() / (t_arrived - t0) / 1e9),
+            "sustained_gbps": float(sizes.sum() / dt / 1e9),
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "ratio": out_bytes / float(sizes.sum()), "device_steps": int(stats[2]),
             "input_bytes": int(sizes.sum())}
