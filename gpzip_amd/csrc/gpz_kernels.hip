@@ -686,9 +686,11 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                         uint32_t *q = kq + (tid & ~63u);
                         const uint32_t ro0 = rnk[2 * tid], ro1 = rnk[2 * tid + 1];
                         const uint32_t tag = k << 24;
+                        // the lane's pairs have queue indices [incl - c, incl): each
+                        // round of 64 takes up where the previous one stopped
+                        uint32_t J0 = I0, J1 = I1, idx = incl - c;
                         for (uint32_t r0 = 0; r0 < tot; r0 += 64) {
-                            uint32_t J0 = I0, J1 = I1, idx = incl - c;
-                            while (J0 | J1) {
+                            while ((J0 | J1) && idx < r0 + 64u) {
                                 uint32_t id, ent;
                                 if (J0) {
                                     const uint32_t m = (1u << __builtin_ctz(J0)) - 1u;
@@ -701,7 +703,7 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                                     ent = (rkk >> 16) + __popc(bk.y & m);
                                     J1 &= J1 - 1u;
                                 }
-                                if (idx - r0 < 64u) q[idx - r0] = id | (ent << 12);
+                                q[idx - r0] = id | (ent << 12);
                                 idx++;
                             }
                             wave_lds_sync();
