@@ -284,6 +284,7 @@ __device__ __forceinline__ uint32_t block_walk(uint32_t J, uint32_t bs, uint32_t
 #pragma unroll
     for (int r = 0; r < 6; r++) {
         const bool inside = j < lim;
+        if (!__ballot(inside)) break;  // every lane has left the block (long matches)
         const int src = inside ? (int)j : (int)lane;
         const uint32_t jj = (uint32_t)__shfl((int)j, src, 64);
         const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)M, src, 64);
