@@ -457,7 +457,7 @@ def main():
     if args.mixed:
         log("mixed sizes: 4096 chunks, 4 KiB - 4 MiB ...")
         mdata, msizes = mixed_sizes_workload()
-        rm = plain_and_batched(torch, ctx, mdata, msizes, max(1, args.steps // 2), 1)
+        rm = plain_and_batched(torch, ctx, mdata, msizes, max(3, args.steps // 2), 1)
         print(json.dumps({"workload": "mixed sizes, 4096 chunks log-uniform 4 KiB-4 MiB, text/binmix/log 1/3 each",
                           "input_bytes": rm["n_in"], "plain_gbps": round(rm["plain"]["gbps"], 3),
                           "batched_gbps": round(rm["batched"]["gbps"], 3),

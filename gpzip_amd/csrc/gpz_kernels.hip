@@ -215,13 +215,15 @@ __device__ __forceinline__ MemberDesc load_member(const MemberDesc *mem, uint32_
     return d;
 }
 
-// Spread consecutive work items of a persistent grid so that each XCD (blocks
-// are dealt round-robin to the 8 XCDs) walks a contiguous range: neighbouring
-// segments share 8/9 of their window, so they should share an L2.
+// Spread consecutive work items so that each XCD (blocks are dealt
+// round-robin to the 8 XCDs) walks a contiguous range: neighbouring segments
+// share 8/9 of their window, so they should share an L2.  XCD x takes the
+// P / 8 (+1 for x < P % 8) items from x * (P / 8) + min(x, P % 8): a bijection
+// of [0, P) for any grid size P.
 __device__ __forceinline__ uint32_t xcd_block() {
-    uint32_t b = blockIdx.x, P = gridDim.x;
-    if ((P & 7u) == 0) return (b & 7u) * (P >> 3) + (b >> 3);
-    return b;
+    const uint32_t b = blockIdx.x, P = gridDim.x;
+    const uint32_t x = b & 7u, q = P >> 3, r = P & 7u;
+    return x * q + min(x, r) + (b >> 3);
 }
 
 // Bytes x..x+3 of an LDS byte array stored as u32 words.
