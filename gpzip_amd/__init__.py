@@ -297,8 +297,10 @@ class Server:
         a = _u8(data)
         _check(self.L.gpz_request_submit(self.h, _ptr(a), a.size, tag))
 
-    def next(self, timeout_ms: int = -1):
-        """(tag, bytes, latency_s) of the next finished request, or None on timeout."""
+    def next(self, timeout_ms: int = -1, copy: bool = True):
+        """(tag, bytes, latency_s) of the next finished request, or None on timeout.
+        With copy=False the result is released unread and its length stands in
+        for the bytes (a load generator that only counts output)."""
         tag = C.c_uint64()
         p = u8p()
         n = C.c_size_t()
@@ -307,7 +309,7 @@ class Server:
         if rc == GPZ_E_AGAIN:
             return None
         _check(rc)
-        data = C.string_at(p, n.value) if n.value else b""
+        data = (C.string_at(p, n.value) if n.value else b"") if copy else n.value
         _check(self.L.gpz_request_release(self.h, tag.value))
         return tag.value, data, lat.value
 

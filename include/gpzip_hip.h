@@ -199,7 +199,7 @@ typedef struct gpz_server gpz_server;
 typedef struct gpz_server_config {
     uint64_t budget_bytes;     /* device-memory budget for admitted work */
     uint32_t max_batch_chunks; /* chunks gathered into one device step */
-    uint32_t max_batch_bytes;  /* input bytes gathered into one device step */
+    uint32_t max_batch_bytes;  /* input bytes gathered into one device step (8 MiB) */
     uint32_t chunk_bytes;      /* request split size (2 MiB) */
     uint32_t streams;          /* device steps in flight */
 } gpz_server_config;
