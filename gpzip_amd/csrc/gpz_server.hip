@@ -164,7 +164,7 @@ void gpz_server_config_default(gpz_server_config *cfg) {
     cfg->max_batch_chunks = 64;
     cfg->max_batch_bytes = 8u << 20;
     cfg->chunk_bytes = 2u << 20;
-    cfg->streams = 8;
+    cfg->streams = 12;
 }
 
 int gpz_server_start(gpz_ctx *ctx, const gpz_server_config *cfg_in, gpz_server **out) {
