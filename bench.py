@@ -287,23 +287,27 @@ def end_to_end(ctx, data: np.ndarray, sizes, expect, threads: int = 16, reps: in
                            "results in host memory (PCIe both ways)"}
 
 
-def profiled_traffic(n_in: int):
-    """HBM bytes per k_match_find launch as last measured with rocprofv3 --pmc
-    (FETCH_SIZE and WRITE_SIZE passes, each scaled by a copy-kernel calibration);
-    the counters cannot be read from inside this process.  None if the
-    committed profile is for another input size."""
+def profiled_counters(n_in: int) -> dict:
+    """k_match_find counter figures as last measured with rocprofv3 --pmc (the
+    counters cannot be read from inside this process): HBM bytes per launch
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by a copy-kernel
+    calibration), VALU issue and lane-use fractions.  Empty if the committed
+    profile is for another input size."""
     path = os.path.join(ROOT, "profiles", "k_match_find_binmix256_traffic.md")
     vals = {}
     try:
         for line in open(path):
             k, _, v = line.partition(":")
-            if k.strip() in ("traffic_bytes_per_launch", "input_bytes"):
-                vals[k.strip()] = int(v.strip())
-    except OSError:
-        return None
-    if vals.get("input_bytes") != n_in or "traffic_bytes_per_launch" not in vals:
-        return None
-    return vals["traffic_bytes_per_launch"]
+            k = k.strip()
+            if k in ("traffic_bytes_per_launch", "input_bytes"):
+                vals[k] = int(v.strip())
+            elif k in ("valu_issue_frac", "valu_lane_frac"):
+                vals[k] = float(v.strip())
+    except (OSError, ValueError):
+        return {}
+    if vals.get("input_bytes") != n_in:
+        return {}
+    return vals
 
 
 def mixed_sizes_workload(seed=3, count=4096):
@@ -431,6 +435,7 @@ def main():
     alg_bytes = r["n_in"] + r["n_out"]  # per launch: sum over members of (n + c)
     k1_s = r["k1_ms"] / 1e3
     achieved = alg_bytes / k1_s / 1e9
+    pc = profiled_counters(r["n_in"])
     line = {
         "metric": "input GB/s on 1 MI355X (% HBM roofline) + ratio; p50/p99 request latency",
         "value": round(r["plain"]["gbps"], 3),
@@ -454,7 +459,8 @@ def main():
                     "vs_plain": round(r["batched"]["gbps"] / r["plain"]["gbps"], 3), "autotune": r["tune"]},
         "roofline": {"bound": "hbm", "kernel": "k_match_find", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
-                     "traffic": profiled_traffic(r["n_in"]), "alg_bytes_per_launch": alg_bytes,
+                     "traffic": pc.get("traffic_bytes_per_launch"), "alg_bytes_per_launch": alg_bytes,
+                     "valu_issue_frac": pc.get("valu_issue_frac"), "valu_lane_frac": pc.get("valu_lane_frac"),
                      "traffic_source": "profiles/k_match_find_binmix256_traffic.md (rocprofv3 --pmc)",
                      "avg_launch_ms": round(r["k1_ms"], 4), "seg_table_ms": round(r["k0_ms"], 4),
                      "emit_ms": round(r["emit_ms"], 4),
