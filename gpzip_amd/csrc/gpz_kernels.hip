@@ -384,6 +384,26 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
 #pragma unroll
         for (int r = 0; r < 4; r++) bm[tid + r * K0_T] = 0;
         __syncthreads();
+        // raw CRC-32 of the segment, started here so that its chain of table
+        // lookups overlaps the atomics and barriers below: 8 bytes per thread,
+        // folded by x^(8 * bytes after them), reduced at the end
+        uint32_t crc = 0;
+        {
+            const uint64_t hi = d2;
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                if (x0 + b < L) {
+                    uint32_t byte = (uint32_t)((b < 8 ? lo : hi) >> (8 * b)) & 0xFF;
+                    crc = crc_tab[(crc ^ byte) & 0xFF] ^ (crc >> 8);
+                }
+            }
+            if (x0 < L) {
+                const uint32_t after = L - min(x0 + 8, L);
+                crc = gf2_mul(crc, L == SEG ? fold : xpow8(after));
+            } else {
+                crc = 0;
+            }
+        }
         uint32_t hh[8];
         uint32_t hvm = 0;
 #pragma unroll
@@ -432,23 +452,8 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
         }
         __syncthreads();
         for (uint32_t id = tid; id < D; id += K0_T) T[TAB_ENT + id] = (fst[id] << 12) | lst[id];
-        // raw CRC-32 of the segment: 8 bytes per thread, folded by x^(8 * bytes after them)
         {
-            uint32_t c = 0;
-            const uint64_t hi = d2;
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                if (x0 + b < L) {
-                    uint32_t byte = (uint32_t)((b < 8 ? lo : hi) >> (8 * b)) & 0xFF;
-                    c = crc_tab[(c ^ byte) & 0xFF] ^ (c >> 8);
-                }
-            }
-            if (x0 < L) {
-                const uint32_t after = L - min(x0 + 8, L);
-                c = gf2_mul(c, L == SEG ? fold : xpow8(after));
-            } else {
-                c = 0;
-            }
+            uint32_t c = crc;
 #pragma unroll
             for (int d = 32; d >= 1; d >>= 1) c ^= __shfl_xor(c, d, 64);
             if ((tid & 63) == 0) tmp[tid >> 6] = c;
