@@ -1641,7 +1641,25 @@ __global__ __launch_bounds__(BT) void k_emit(Work w) {
         // tokens, 4096 per tile
         const uint32_t nt = w.ntok[m];
         const uint32_t *tk = w.walked + md.pos_base;
+        // this thread's 4 tokens of a tile (member regions are 64-position
+        // aligned, so a full group is one 16-byte load), loaded one tile ahead
+        auto load4 = [&](uint32_t t0) {
+            const uint32_t ti = t0 + tid * 4;
+            uint4 q = make_uint4(0, 0, 0, 0);
+            if (ti + 4 <= nt) {
+                q = *reinterpret_cast<const uint4 *>(tk + ti);
+            } else {
+                if (ti < nt) q.x = tk[ti];
+                if (ti + 1 < nt) q.y = tk[ti + 1];
+                if (ti + 2 < nt) q.z = tk[ti + 2];
+            }
+            return q;
+        };
+        uint4 nxt = load4(0);
         for (uint32_t t0 = 0; t0 < nt; t0 += 4 * BT) {
+            const uint4 cur = nxt;
+            nxt = load4(t0 + 4 * BT);
+            const uint32_t tq[4] = {cur.x, cur.y, cur.z, cur.w};
             uint32_t v[4][4], nb[4][4];
             uint32_t mybits = 0;
 #pragma unroll
@@ -1653,7 +1671,7 @@ __global__ __launch_bounds__(BT) void k_emit(Work w) {
                     nb[k][f] = 0;
                 }
                 if (ti >= nt) continue;
-                uint32_t t = tk[ti], len = t >> 16, x = t & 0xFFFFu;
+                uint32_t t = tq[k], len = t >> 16, x = t & 0xFFFFu;
                 if (len == 0) {
                     uint32_t c = ctab[x & 0xFF];
                     v[k][0] = c & 0xFFFF;
