@@ -167,7 +167,7 @@ def plain_and_batched(torch, ctx, data: np.ndarray, sizes, steps, warmup, check_
     full = b"".join(chunks_out)
     assert gzip.decompress(full) == data[:n_in].tobytes(), "output does not decode to the input"
     if check_oracle:
-        from oracle import oracle as O  # checker only
+        from oracle import oracle as O  # expected outputs only
 
         src = 0
         for i, s in enumerate(sizes[:check_oracle]):

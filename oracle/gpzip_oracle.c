@@ -1,8 +1,8 @@
 /*
  * gpzip_oracle.c -- CPU restatement of gpzip's GPU gzip compress path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the CPU
- * baseline that bench.py reports.  The product never links or calls it.
+ * TEST INFRASTRUCTURE ONLY: the expected outputs for the HIP library's
+ * parity tests and the CPU baseline that bench.py reports.  The product never links or calls it.
  *
  * The reference is Rust + WGSL on wgpu and cannot be built here, so this
  * file restates its algorithm in plain C, one function per reference

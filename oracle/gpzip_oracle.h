@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (gpzip_amd/, the HIP
  * library, include/gpzip_hip.h) links, loads or calls this code.  Only the
- * tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as
- * the checker and as the reported CPU baseline.
+ * tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, for
+ * the expected outputs and as the reported CPU baseline.
  *
  * Every function names the reference file:line it restates (paths relative
  * to the reference repository, crates/gpzip- prefix dropped).

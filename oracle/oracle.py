@@ -1,6 +1,7 @@
 """ctypes binding of the CPU oracle (liboracle.so) and the zlib CPU baseline
-(libcpuzlib.so).  TEST AND BENCH INFRASTRUCTURE ONLY: the tests use it as the
-checker, bench.py's cpu_baseline leg times it; the product never imports it.
+(libcpuzlib.so).  TEST AND BENCH INFRASTRUCTURE ONLY: the tests compare the HIP
+library against it, bench.py's cpu_baseline leg times it; the product never
+imports it.
 """
 from __future__ import annotations
 

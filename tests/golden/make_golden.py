@@ -71,7 +71,7 @@ def construct(n: int, plants: dict[int, bytes], query_hashes_from: list[int], se
 
 def match_kats():
     kats = []
-    # survey KATs: q x 64 -> [lit q, (64-p, p) for 1<=p<=61, lit, lit].  Every
+    # hand-derived KATs: q x 64 -> [lit q, (64-p, p) for 1<=p<=61, lit, lit].  Every
     # position hashes alike (one segment): oldest = 0, newest = 61; newest >= p
     # is rejected for p <= 61 so the oldest candidate (p=0) gives distance p.
     q = b"q" * 64
