@@ -51,6 +51,7 @@ struct Request {
     clk::time_point t_submit, t_done;
     uint64_t charge = 0;
     int status = GPZ_OK;
+    bool handed_out = false;  // returned by gpz_request_next: only then may it be released
 };
 
 struct ChunkRef {
@@ -244,6 +245,7 @@ int gpz_request_next(gpz_server *s, int timeout_ms, uint64_t *tag, const uint8_t
         return GPZ_E_AGAIN;
     Request *r = s->finished.front();
     s->finished.pop_front();
+    r->handed_out = true;
     *tag = r->tag;
     *data = r->out.data();
     *len = r->out.size();
@@ -256,7 +258,7 @@ int gpz_request_release(gpz_server *s, uint64_t tag) {
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->live.find(tag);
     if (it == s->live.end()) return fail(GPZ_E_INVAL, "unknown tag");
-    if (it->second->done_chunks != it->second->nchunks) return fail(GPZ_E_INVAL, "request still running");
+    if (!it->second->handed_out) return fail(GPZ_E_INVAL, "request not yet returned by gpz_request_next");
     s->live.erase(it);
     return GPZ_OK;
 }

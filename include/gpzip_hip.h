@@ -214,6 +214,8 @@ int gpz_request_submit(gpz_server *s, const uint8_t *in, size_t len, uint64_t ta
  * owned by the server until gpz_request_release(tag). */
 int gpz_request_next(gpz_server *s, int timeout_ms, uint64_t *tag, const uint8_t **data,
                      size_t *len, double *latency_s);
+/* Free a request that gpz_request_next returned; GPZ_E_INVAL for a tag it has
+ * not returned yet (the server may still be writing that request's output). */
 int gpz_request_release(gpz_server *s, uint64_t tag);
 /* Counters: [0] requests done, [1] input bytes done, [2] device steps,
  * [3] peak admitted bytes. */

@@ -247,6 +247,9 @@ def test_server_alone_equals_mix(ctx):
     try:
         for i, r in enumerate(reqs):
             srv.submit(r, i)
+            if i == 0:
+                # a tag not yet returned by gpz_request_next cannot be released
+                assert ctx.L.gpz_request_release(srv.h, 0) != 0
             tag, data, lat = srv.next()
             assert tag == i
             alone[i] = data
