@@ -268,6 +268,34 @@ def test_server_alone_equals_mix(ctx):
         assert gzip.decompress(alone[i]) == r.tobytes()
 
 
+def test_random_chunks_equal_oracle(ctx):
+    """Seeded random chunks (sizes log-uniform over 1 B - 1 MiB, the synthetic
+    corpora, uniform random bytes, zeros, and splices of two kinds) through
+    the plain and the batched form, each equal to the oracle."""
+    rng = np.random.default_rng(2024)
+    kinds = ["text", "binmix", "log", "random", "zeros"]
+    chunks = []
+    for _ in range(60):
+        n = int(np.exp(rng.uniform(0, np.log(1 << 20))))
+        k = kinds[rng.integers(len(kinds))]
+        if k == "random":
+            c = rng.integers(0, 256, n, dtype=np.uint8)
+        elif k == "zeros":
+            c = np.zeros(n, np.uint8)
+        else:
+            c = gpzip_amd.synth(k, n, int(rng.integers(1 << 30)))
+        if n > 16 and rng.random() < 0.25:
+            cut = int(rng.integers(1, n))
+            c = np.concatenate([c[:cut], gpzip_amd.synth(kinds[rng.integers(3)], n - cut, int(rng.integers(1 << 30)))])
+        chunks.append(c)
+    ref = [O.gzip_chunk(c) for c in chunks]
+    say("random chunks batched")
+    assert ctx.batch(chunks) == ref
+    say("random chunks plain")
+    for c, r in zip(chunks, ref):
+        assert ctx.compress_chunk(c) == r, c.size
+
+
 def test_determinism_repeat(ctx):
     data = gpzip_amd.synth("binmix", 1 << 20, 77)
     say("determinism")
