@@ -395,6 +395,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mixed", action="store_true")
+    ap.add_argument("--corpus", nargs="+", default=[], metavar="KIND:MIB",
+                    help="also run the plain and batched forms on KIND (text, binmix, log) of MIB MiB in "
+                         "2 MiB chunks, e.g. text:16 log:1024")
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-submitters", type=int, default=4, help="threads sharing the arrival schedule")
@@ -475,6 +478,16 @@ def main():
         line["cpu_zlib"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in zl.items()}
     print(json.dumps(line), flush=True)
 
+    for spec in args.corpus:
+        kind, mib = spec.split(":")
+        n = int(mib) << 20
+        log(f"corpus {kind}, {mib} MiB ...")
+        cdata = gpzip_amd.synth(kind, n, 1)
+        csizes = [min(2 * MiB, n - o) for o in range(0, n, 2 * MiB)]
+        rc = plain_and_batched(torch, ctx, cdata, csizes, max(3, args.steps // 2), 1, check_oracle=min(8, len(csizes)))
+        print(json.dumps({"workload": f"{kind} {mib} MiB, 2 MiB chunks", "input_bytes": rc["n_in"],
+                          "plain_gbps": round(rc["plain"]["gbps"], 3), "batched_gbps": round(rc["batched"]["gbps"], 3),
+                          "ratio": round(rc["ratio"], 5), "autotune": rc["tune"]}), flush=True)
     if args.mixed:
         log("mixed sizes: 4096 chunks, 4 KiB - 4 MiB ...")
         mdata, msizes = mixed_sizes_workload()
