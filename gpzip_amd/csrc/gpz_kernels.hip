@@ -540,10 +540,14 @@ constexpr int WIN_WORDS = 9344;  // 37,376 bytes: 32 KiB back + 4 KiB segment + 
 // differ).  Returns the exact length, capped at lim = min(258, n - p), when it
 // is below 16 or lim <= 16; otherwise 16 | PEND: the length is at least 16 and
 // its exact value is left to ext_len.  own0 / own1 hold bytes p .. p+7.
+// FAST: the caller knows the candidate lies before p within the window and
+// that lim is 258, so the range checks and the limit compares fold away.
 constexpr uint32_t PEND = 0x10000u;
-__device__ __forceinline__ uint32_t cand_short(const uint32_t *win, uint32_t WB, uint32_t lim, uint32_t p,
+template <bool FAST>
+__device__ __forceinline__ uint32_t cand_short(const uint32_t *win, uint32_t WB, uint32_t lim_in, uint32_t p,
                                                uint32_t cand, uint32_t window, uint32_t own0, uint32_t own1) {
-    if (cand >= p || p - cand > window) return 0;
+    const uint32_t lim = FAST ? MAX_MATCH : lim_in;
+    if (!FAST && (cand >= p || p - cand > window)) return 0;
     const uint32_t c = cand - WB, ic = c >> 2, sc = c & 3u;
     const uint32_t w0 = win[ic], w1 = win[ic + 1], w2 = win[ic + 2];
     const uint32_t x0 = own0 ^ __builtin_amdgcn_alignbyte(w1, w0, sc);
@@ -738,17 +742,19 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
 #pragma unroll
             for (int i = 0; i < 4; i++)
                 fv[i] = (((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH) ? fl[qid[i]] : EMPTY;
+            auto pass1 = [&](auto fast) {
+            constexpr bool FAST = decltype(fast)::value;
 #pragma unroll 1
             for (uint32_t i = 0; i < 4; i++) {
                 const uint32_t v = fv[0];
                 uint32_t cs = st[0], pend = 0;
                 if ((v >> 24) == k) {
                     const uint32_t p = S + x0 + i;
-                    const uint32_t lim = min(n - p, MAX_MATCH);
+                    const uint32_t lim = FAST ? MAX_MATCH : min(n - p, MAX_MATCH);
                     const uint32_t own0 = __builtin_amdgcn_alignbyte(own_hi, own_lo, i);
                     const uint32_t own1 = __builtin_amdgcn_alignbyte(own_w2, own_hi, i);
                     const uint32_t cn = base + (v & 0xFFFu), co = base + ((v >> 12) & 0xFFFu);
-                    const uint32_t ln = cand_short(win, WB, lim, p, cn, w.window, own0, own1);
+                    const uint32_t ln = cand_short<FAST>(win, WB, lim, p, cn, w.window, own0, own1);
                     if (ln & PEND) {
                         pend = 0x80000000u | (cn - WB);
                     } else {
@@ -757,7 +763,7 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                         // (lz77_hash_lookup.wgsl:102); equal to the newest it
                         // cannot give a strictly longer match
                         if ((cs >> 16) < OLDEST_SKIP && co != cn) {
-                            const uint32_t lo = cand_short(win, WB, lim, p, co, w.window, own0, own1);
+                            const uint32_t lo = cand_short<FAST>(win, WB, lim, p, co, w.window, own0, own1);
                             if (lo & PEND) pend = 0xC0000000u | (co - WB);
                             else if (lo > (cs >> 16)) cs = (lo << 16) | (p - co);
                         }
@@ -767,6 +773,14 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                 st[0] = st[1]; st[1] = st[2]; st[2] = st[3]; st[3] = cs;
                 pd[0] = pd[1]; pd[1] = pd[2]; pd[2] = pd[3]; pd[3] = pend;
             }
+            };
+            // interior segments (every match limit is 258) and back segments 1 .. 7
+            // (candidates before the position, within the window) take the
+            // check-free pass
+            if (E + MAX_MATCH <= n && k >= 1 && 4095u + k * SEG <= w.window)
+                pass1(std::integral_constant<bool, true>{});
+            else
+                pass1(std::integral_constant<bool, false>{});
             {
                 // pass 2 (wave-local: no workgroup barrier, the trip count differs per wave)
                 const uint32_t lane = tid & 63u;
