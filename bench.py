@@ -181,7 +181,7 @@ def plain_and_batched(torch, ctx, data: np.ndarray, sizes, steps, warmup, check_
 
     # dominant-kernel timing: batched plan, direct launches on its stream, HIP
     # events bracket the match-find and emit kernels
-    tplan = DevicePlan(ctx, list(zip(offs, sizes)), gpzip_amd.GPZ_PLAN_PERSISTENT)
+    tplan = DevicePlan(ctx, list(zip(offs, sizes)), gpzip_amd.GPZ_PLAN_PERSISTENT | gpzip_amd.GPZ_PLAN_TIMED)
     d_out_t = torch.zeros(tplan.out_cap + 256, dtype=torch.uint8, device="cuda:0")
     tplan.run(base_in, d_out_t.data_ptr())
     tplan.sync()

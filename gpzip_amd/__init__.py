@@ -17,7 +17,7 @@ HIP_LIB_PATH = os.path.join(HERE, "libgpzip_hip.so")
 SYNTH_LIB_PATH = os.path.join(HERE, "libgpzip_synth.so")
 
 GPZ_OK, GPZ_E_NODEV, GPZ_E_OOM, GPZ_E_INVAL, GPZ_E_DEVICE, GPZ_E_NOSPACE, GPZ_E_BUSY, GPZ_E_AGAIN = range(8)
-GPZ_PLAN_GRAPH, GPZ_PLAN_PERSISTENT = 1, 2
+GPZ_PLAN_GRAPH, GPZ_PLAN_PERSISTENT, GPZ_PLAN_TIMED = 1, 2, 4
 
 # Every entry point include/gpzip_hip.h declares.
 ABI_SYMBOLS = [

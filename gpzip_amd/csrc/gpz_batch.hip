@@ -252,6 +252,8 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
     if (!ctx || !out || (n && !descs)) return fail(GPZ_E_INVAL, "bad argument");
     *out = nullptr;
     if (n == 0) return fail(GPZ_E_INVAL, "empty plan");
+    if (flags & ~(uint32_t)(GPZ_PLAN_GRAPH | GPZ_PLAN_PERSISTENT | GPZ_PLAN_TIMED))
+        return fail(GPZ_E_INVAL, "unknown plan flag");
     std::vector<uint64_t> lens(n), offs(n);
     for (size_t i = 0; i < n; i++) {
         lens[i] = descs[i].len;
@@ -300,7 +302,7 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
     clear_error();
     if (!ctx || !out || !d_in || !d_out || (n && !descs)) return fail(GPZ_E_INVAL, "bad argument");
     gpz_plan *p = nullptr;
-    int rc = gpz_plan_create(ctx, descs, n, GPZ_PLAN_PERSISTENT, nullptr, &p);
+    int rc = gpz_plan_create(ctx, descs, n, GPZ_PLAN_PERSISTENT | GPZ_PLAN_TIMED, nullptr, &p);
     if (rc) return rc;
     const uint32_t cu = (uint32_t)ctx_cus(ctx);
     const uint32_t G = std::max<uint32_t>(1, p->pl.G), M = std::max<uint32_t>(1, p->pl.M);
@@ -435,12 +437,14 @@ int gpz_plan_run(gpz_plan *p, const void *d_in, void *d_out) {
             p->bound_in = d_in;
             p->bound_out = d_out;
         }
-        HIPCHK(hipEventRecord(ln.ev[0], ln.st));
+        const bool ev = p->flags & GPZ_PLAN_TIMED;
+        if (ev) HIPCHK(hipEventRecord(ln.ev[0], ln.st));
         HIPCHK(hipGraphLaunch(ln.graph, ln.st));
-        HIPCHK(hipEventRecord(ln.ev[2], ln.st));
+        if (ev) HIPCHK(hipEventRecord(ln.ev[2], ln.st));
         p->timed = false;
     } else {
-        enqueue(w, p->grids, MODE_FULL, ln.st, ln.ev);
+        // events only when asked for: each one is a marker on the queue
+        enqueue(w, p->grids, MODE_FULL, ln.st, (p->flags & GPZ_PLAN_TIMED) ? ln.ev : nullptr);
         HIPCHK(hipGetLastError());
         p->timed = true;
     }
@@ -466,6 +470,7 @@ int gpz_plan_result(gpz_plan *p, uint64_t *chunk_off, uint64_t *chunk_len, uint6
 
 int gpz_plan_timing(gpz_plan *p, float *ms, size_t n) {
     if (!p || !ms) return fail(GPZ_E_INVAL, "bad argument");
+    if (!(p->flags & GPZ_PLAN_TIMED)) return fail(GPZ_E_INVAL, "plan created without GPZ_PLAN_TIMED");
     Lane &ln = p->lane;
     float v[4] = {0, 0, 0, 0};
     HIPCHK(hipEventElapsedTime(&v[0], ln.ev[0], ln.ev[2]));

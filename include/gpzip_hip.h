@@ -143,7 +143,10 @@ void gpz_batch_free(gpz_batch *b);
 
 typedef struct gpz_plan gpz_plan;
 
-enum { GPZ_PLAN_GRAPH = 1u, GPZ_PLAN_PERSISTENT = 2u };
+/* GPZ_PLAN_TIMED records HIP events around the kernels of every run, for
+ * gpz_plan_timing.  The events are not free: on 2 MiB chunks they cost about
+ * 15 % of the pipeline, so plans that are not being profiled leave it off. */
+enum { GPZ_PLAN_GRAPH = 1u, GPZ_PLAN_PERSISTENT = 2u, GPZ_PLAN_TIMED = 4u };
 
 /* Launch shape of the batched form.  Zero fields keep the built-in choice. */
 typedef struct gpz_tune {
@@ -188,7 +191,8 @@ int gpz_plan_sync(gpz_plan *p);
 int gpz_plan_result(gpz_plan *p, uint64_t *chunk_off, uint64_t *chunk_len, uint64_t *total);
 /* Device time (ms) of the last run, from HIP events on the plan's stream:
  * [0] whole run, [1] match-find kernel, [2] emit kernel, [3] segment-table
- * kernel ([1]-[3] only for runs without GPZ_PLAN_GRAPH). */
+ * kernel ([1]-[3] only for runs without GPZ_PLAN_GRAPH).  GPZ_E_INVAL unless
+ * the plan was created with GPZ_PLAN_TIMED. */
 int gpz_plan_timing(gpz_plan *p, float *ms, size_t n);
 void gpz_plan_destroy(gpz_plan *p);
 

@@ -237,6 +237,57 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
             ctx.set_tune(bad)
 
 
+def test_plan_flags_and_timing(ctx):
+    """Direct, timed and graph runs of one plan layout give the oracle's bytes;
+    per-kernel timing needs GPZ_PLAN_TIMED; unknown flags are rejected."""
+    import torch
+
+    chunks = [gpzip_amd.synth(k, s, 70 + i) for i, (k, s) in enumerate([("binmix", 2 << 20), ("text", 300001)])]
+    ref = [O.gzip_chunk(c) for c in chunks]
+    offs, o = [], 0
+    for c in chunks:
+        offs.append(o)
+        o += (c.size + 255) // 256 * 256
+    host = np.zeros(o + 256, np.uint8)
+    for off, c in zip(offs, chunks):
+        host[off:off + c.size] = c
+    d_in = torch.from_numpy(host).to("cuda:0")
+    arr = (gpzip_amd.Desc * len(chunks))()
+    for i, (a, c) in enumerate(zip(offs, chunks)):
+        arr[i].offset, arr[i].len = a, c.size
+    L = ctx.L
+    G, P, T = gpzip_amd.GPZ_PLAN_GRAPH, gpzip_amd.GPZ_PLAN_PERSISTENT, gpzip_amd.GPZ_PLAN_TIMED
+    h = gpzip_amd.C.c_void_p()
+    assert L.gpz_plan_create(ctx.h, arr, len(chunks), 8, None, gpzip_amd.C.byref(h)) != 0
+    for flags in (0, T, P | T, G, G | P | T):
+        say("plan flags", flags)
+        h = gpzip_amd.C.c_void_p()
+        gpzip_amd._check(L.gpz_plan_create(ctx.h, arr, len(chunks), flags, None, gpzip_amd.C.byref(h)))
+        try:
+            d_out = torch.zeros(L.gpz_plan_out_capacity(h) + 256, dtype=torch.uint8, device="cuda:0")
+            for _ in range(2):
+                gpzip_amd._check(L.gpz_plan_run(h, gpzip_amd.C.c_void_p(d_in.data_ptr()),
+                                                gpzip_amd.C.c_void_p(d_out.data_ptr())))
+                gpzip_amd._check(L.gpz_plan_sync(h))
+            off = np.zeros(len(chunks), np.uint64)
+            ln = np.zeros(len(chunks), np.uint64)
+            tot = gpzip_amd.C.c_uint64()
+            gpzip_amd._check(L.gpz_plan_result(h, off.ctypes.data_as(gpzip_amd.u64p),
+                                               ln.ctypes.data_as(gpzip_amd.u64p), gpzip_amd.C.byref(tot)))
+            out = d_out.cpu().numpy()
+            assert [out[int(a):int(a) + int(b)].tobytes() for a, b in zip(off, ln)] == ref
+            ms = (gpzip_amd.C.c_float * 4)()
+            rc = L.gpz_plan_timing(h, ms, 4)
+            if flags & T:
+                assert rc == 0 and ms[0] > 0
+                if not flags & G:
+                    assert 0 < ms[1] < ms[0] and 0 < ms[2] < ms[0] and 0 < ms[3] < ms[0]
+            else:
+                assert rc != 0
+        finally:
+            L.gpz_plan_destroy(h)
+
+
 def test_server_alone_equals_mix(ctx):
     kinds = ["text", "binmix", "log"]
     reqs = [gpzip_amd.synth(kinds[i % 3], s, 70 + i)
