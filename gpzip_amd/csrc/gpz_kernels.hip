@@ -605,6 +605,11 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
     __shared__ uint32_t fl[SEG];     // per hash id: tag << 24 | first << 12 | last
     __shared__ uint32_t tmp[32];
     __shared__ uint32_t kq[K1_T];    // per wave: 64-entry queue of pending candidates
+    // Workgroup votes, one word each: [k] back segment k, [15] the walk summary.  A
+    // wave with a set predicate writes 1 before a barrier; everyone reads after it.
+    // (__syncthreads_or builds a flat work-item id from the y/z ids and the dispatch
+    // packet's sizes; at 64 VGPRs that state spilled and cost 4 % of this kernel.)
+    __shared__ uint32_t kvote[16];
     const uint32_t tid = threadIdx.x;
     const uint32_t G = uni(w.counts->nseg);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
@@ -646,6 +651,7 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
             bits[2 * tid] = bw.x;
             bits[2 * tid + 1] = bw.y;
             reinterpret_cast<uint32_t *>(rnk)[tid] = T[TAB_RANK + tid];
+            if (tid < 16) kvote[tid] = 0;
         }
         __syncthreads();
 
@@ -824,7 +830,9 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
             int active = 0;
 #pragma unroll
             for (int i = 0; i < 4; i++) active |= ((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH;
-            if (!__syncthreads_or(active)) break;
+            if (__ballot(active) && (tid & 63u) == 0) kvote[k] = 1u;
+            __syncthreads();
+            if (!kvote[k]) break;
         }
 
         // ---- tokens
@@ -854,8 +862,9 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                 tokL[x0 + i] = t4[i];
                 anym |= (x0 + i < L) && (t4[i] >> 16) != 0;
             }
-            const int any_match = __syncthreads_or(anym);
-            seg_summary(w, g, L, E == n, any_match != 0, tokL, win + SEG, fl);
+            if (__ballot(anym) && (tid & 63u) == 0) kvote[15] = 1u;
+            __syncthreads();
+            seg_summary(w, g, L, E == n, kvote[15] != 0, tokL, win + SEG, fl);
         }
         __syncthreads();
     }
