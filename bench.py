@@ -8,13 +8,12 @@ one pass over the whole corpus.  Beside it: the batched form (one graph replay
 over all 128 chunks) on the same data, the roofline of the dominant kernel
 (match-find) from HIP events on its stream, the end-to-end rates through the
 drop-in boundary with host buffers (PCIe both ways; never the headline), and
-the CPU baselines.
+the CPU baseline.  The same JSON line carries the mixed-size comparison
+(4,096 chunks of 4 KiB-4 MiB, batched form against the plain form on the same
+GPU) and the serving form (20,000 Poisson requests at a stated offered load:
+p50/p99 latency and sustained GB/s).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mixed] [--serve]
-
---mixed adds the mixed-size batched-vs-plain comparison (4,096 log-uniform
-chunks of 4 KiB-4 MiB); --serve adds the serving form (Poisson requests).
-Both are printed as extra JSON lines after the headline line.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-mixed] [--no-serve]
 """
 from __future__ import annotations
 
@@ -33,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import gpzip_amd  # noqa: E402
+from gpzip_amd import workloads  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
 MiB = 1 << 20
@@ -208,31 +208,63 @@ def plain_and_batched(torch, ctx, data: np.ndarray, sizes, steps, warmup, check_
     return res
 
 
-def cpu_baselines(data: np.ndarray, sample_bytes: int, chunk: int):
+def host_cores() -> tuple[int, dict]:
+    """Cores this process may use: its CPU affinity, capped by a cgroup CPU
+    quota when one is set (a share of a larger host)."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = max(1, int(int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    n = min(aff, quota) if quota else aff
+    return n, {"affinity_cpus": aff, "cgroup_cpu_quota": quota}
+
+
+def cpu_baseline(data: np.ndarray, chunk: int, reps: int = 3):
+    """gpzip-codec-cpu's structure (2 MiB chunks, one zlib level-5 gzip member
+    each, in chunk order, all host cores) on the whole workload, median of
+    `reps` runs; and one core on the first 32 MiB."""
     from oracle import oracle as O  # CPU baseline only
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    sample = data[:sample_bytes]
-    pieces = [np.ascontiguousarray(sample[i:i + chunk]) for i in range(0, sample.size, chunk)]
-    t0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(threads) as ex:
-        outs = list(ex.map(O.gzip_chunk_into, pieces))
-    dt = time.perf_counter() - t0
-    port = {"value": sample.size / dt / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample.size // MiB} MiB of the workload ({len(pieces)} x {chunk // MiB} MiB chunks), "
-                      f"oracle restatement of the GPU algorithm, one chunk per thread",
-            "ratio": sum(outs) / sample.size}
-    t0 = time.perf_counter()
-    ztot, _ = O.zlib_chunks(data, chunk, 5, threads)
-    dz = time.perf_counter() - t0
-    one = data[:16 * MiB]
+    cores, info = host_cores()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ztot, _ = O.zlib_chunks(data, chunk, 5, cores)
+        times.append(time.perf_counter() - t0)
+    dz = float(np.median(times))
+    one = data[:32 * MiB]
     t0 = time.perf_counter()
     O.zlib_chunks(one, chunk, 5, 1)
     d1 = time.perf_counter() - t0
-    zl = {"value": data.size / dz / 1e9, "unit": "GB/s", "cores": threads,
-          "kind": "gpzip-codec-cpu structure on system zlib " + O.zlib_version() + " level 5 (not zlib-rs)",
-          "ratio": ztot / data.size, "single_core_gbps": one.size / d1 / 1e9}
-    return port, zl
+    return {"value": data.size / dz / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"the whole workload ({data.size // MiB} MiB, {data.size // chunk} x {chunk // MiB} MiB chunks), "
+                      f"median of {reps} runs",
+            "program": "gpzip-codec-cpu structure (codec-cpu/src/compress.rs:52-59, backend.rs:14-35): one gzip "
+                       "member per 2 MiB chunk, zlib level 5, chunk order, on system zlib "
+                       + O.zlib_version() + " (not zlib-rs 0.6.3; member bytes unpinned)",
+            "ratio": ztot / data.size, "single_core_gbps": one.size / d1 / 1e9, **info}
+
+
+def cpu_port(data: np.ndarray, sample_bytes: int, chunk: int):
+    """The oracle (a CPU restatement of the GPU algorithm) on the host cores."""
+    from oracle import oracle as O  # CPU baseline only
+
+    cores, _ = host_cores()
+    sample = data[:sample_bytes]
+    pieces = [np.ascontiguousarray(sample[i:i + chunk]) for i in range(0, sample.size, chunk)]
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        outs = list(ex.map(O.gzip_chunk_into, pieces))
+    dt = time.perf_counter() - t0
+    return {"value": sample.size / dt / 1e9, "unit": "GB/s", "cores": cores,
+            "sample": f"first {sample.size // MiB} MiB of the workload ({len(pieces)} x {chunk // MiB} MiB chunks)",
+            "program": "the oracle: CPU restatement of the GPU algorithm (segmented hash, lazy walk, "
+                       "encode_block_fast), one chunk per thread",
+            "ratio": sum(outs) / sample.size}
 
 
 def end_to_end(ctx, data: np.ndarray, sizes, expect, threads: int = 16, reps: int = 3):
@@ -287,12 +319,20 @@ def end_to_end(ctx, data: np.ndarray, sizes, expect, threads: int = 16, reps: in
                            "results in host memory (PCIe both ways)"}
 
 
+def kernel_source_hash() -> str:
+    import hashlib
+
+    with open(os.path.join(ROOT, "gpzip_amd", "csrc", "gpz_kernels.hip"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
 def profiled_counters(n_in: int) -> dict:
     """k_match_find counter figures as last measured with rocprofv3 --pmc (the
     counters cannot be read from inside this process): HBM bytes per launch
     (FETCH_SIZE and WRITE_SIZE passes, each scaled by a copy-kernel
     calibration), VALU issue and lane-use fractions.  Empty if the committed
-    profile is for another input size."""
+    profile is for another input size or for other kernel sources (its
+    kernel_source_sha256 line differs from the current gpz_kernels.hip)."""
     path = os.path.join(ROOT, "profiles", "k_match_find_binmix256_traffic.md")
     vals = {}
     try:
@@ -303,45 +343,28 @@ def profiled_counters(n_in: int) -> dict:
                 vals[k] = int(v.strip())
             elif k in ("valu_issue_frac", "valu_lane_frac"):
                 vals[k] = float(v.strip())
+            elif k == "kernel_source_sha256":
+                vals[k] = v.strip()
     except (OSError, ValueError):
         return {}
-    if vals.get("input_bytes") != n_in:
+    if vals.get("input_bytes") != n_in or vals.get("kernel_source_sha256") != kernel_source_hash():
         return {}
     return vals
 
 
-def mixed_sizes_workload(seed=3, count=4096):
-    sizes = gpzip_amd.loguniform_sizes(seed, count, 4096, 4 << 20).astype(np.int64)
-    choose = gpzip_amd.uniform(seed + 1, count)
-    kinds = ["text", "binmix", "log"]
-    # content per chunk drawn 1/3 each from the three generators; draw from three
-    # shared pools so the 2.5 GiB does not have to be generated chunk by chunk
-    pool_n = int(sizes.max()) + (256 << 20)
-    pools = [gpzip_amd.synth(k, pool_n, 10 + i) for i, k in enumerate(kinds)]
-    offs = (gpzip_amd.uniform(seed + 2, count) * (pool_n - sizes)).astype(np.int64) // 16 * 16
-    parts = [pools[int(c * 3)][o:o + s] for c, o, s in zip(choose, offs, sizes)]
-    return np.concatenate(parts), [int(s) for s in sizes]
-
-
-def serve(torch, ctx, n_requests, load_gbps, seed=4, submitters=4, **server_cfg):
+def serve(ctx, n_requests, load_gbps, seed=4, submitters=4, budget_bytes=64 * 10**9, **server_cfg):
     """Config 4: Poisson arrivals at load_gbps, log-normal request sizes.  The
     schedule is shared by `submitters` threads (request i goes to thread
     i % submitters, each sleeping until its arrival time), so one caller's
     copy into the server does not cap the offered rate; the drain counts
     output bytes without copying them."""
-    sizes = gpzip_amd.lognormal_sizes(seed, n_requests, 256 * 1024, 2.0, 1024, 64 << 20).astype(np.int64)
-    kinds = ["text", "binmix", "log"]
-    pool_n = (64 << 20) + (192 << 20)
-    pools = [gpzip_amd.synth(k, pool_n, 20 + i) for i, k in enumerate(kinds)]
-    choose = gpzip_amd.uniform(seed + 1, n_requests)
-    offs = (gpzip_amd.uniform(seed + 2, n_requests) * (pool_n - sizes)).astype(np.int64)
-    gaps = -np.log(1.0 - gpzip_amd.uniform(seed + 3, n_requests)) * (sizes.mean() / (load_gbps * 1e9))
-    arrive = np.cumsum(gaps)
-    srv = gpzip_amd.Server(ctx, budget_bytes=64 * 10**9, **server_cfg)
+    import threading
+
+    R = workloads.ServingRequests(n_requests, load_gbps, seed)
+    srv = gpzip_amd.Server(ctx, budget_bytes=budget_bytes, **server_cfg)
     lat = []
     done = 0
     out_bytes = 0
-    import threading
 
     def drain():
         nonlocal done, out_bytes
@@ -359,10 +382,9 @@ def serve(torch, ctx, n_requests, load_gbps, seed=4, submitters=4, **server_cfg)
     def submit(t):
         for i in range(t, n_requests, submitters):
             now = time.perf_counter()
-            if t0 + arrive[i] > now:
-                time.sleep(t0 + arrive[i] - now)
-            src = pools[int(choose[i] * 3)]
-            srv.submit(src[offs[i]:offs[i] + sizes[i]], i)
+            if t0 + R.arrive[i] > now:
+                time.sleep(t0 + R.arrive[i] - now)
+            srv.submit(R.data(i), i)
         t_last[t] = time.perf_counter()
 
     th = threading.Thread(target=drain)
@@ -379,14 +401,35 @@ def serve(torch, ctx, n_requests, load_gbps, seed=4, submitters=4, **server_cfg)
     stats = srv.stats()
     srv.stop()
     lat = np.array(lat)
+    total = int(R.sizes.sum())
     # arrived_gbps: the rate the submitters actually reached; below offered_gbps when
     # submission itself fell behind the schedule
     return {"requests": n_requests, "offered_gbps": load_gbps,
-            "arrived_gbps": float(sizes.sum() / (t_arrived - t0) / 1e9),
-            "sustained_gbps": float(sizes.sum() / dt / 1e9),
-            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-            "ratio": out_bytes / float(sizes.sum()), "device_steps": int(stats[2]),
-            "input_bytes": int(sizes.sum()), "submitters": submitters}
+            "arrived_gbps": round(total / (t_arrived - t0) / 1e9, 3),
+            "sustained_gbps": round(total / dt / 1e9, 3),
+            "p50_ms": round(float(np.percentile(lat, 50) * 1e3), 3),
+            "p99_ms": round(float(np.percentile(lat, 99) * 1e3), 3),
+            "ratio": round(out_bytes / float(total), 5), "device_steps": stats["steps"],
+            "input_bytes": total, "submitters": submitters, "budget_bytes": budget_bytes,
+            "peak_arena_bytes": stats["peak_bytes"], "admissions_waited": stats["waited"]}
+
+
+def mixed_sizes(torch, ctx, steps):
+    """Config 3: the batched form against the plain form on 4,096 chunks of
+    4 KiB-4 MiB (outputs equal chunk by chunk, the whole output decodes, the
+    first chunks equal the oracle)."""
+    mdata, msizes = workloads.mixed_sizes_workload()
+    rm = plain_and_batched(torch, ctx, mdata, msizes, steps, 1, check_oracle=2)
+    n_in, n_out = rm["n_in"], rm["n_out"]
+    return {"workload": "4,096 chunks, sizes log-uniform over 4 KiB-4 MiB (seed 3), content 1/3 each "
+                        "text / binmix / log, inputs resident in HBM",
+            "input_bytes": n_in, "chunks": len(msizes), "steps": steps,
+            "plain_gbps": round(rm["plain"]["gbps"], 3), "batched_gbps": round(rm["batched"]["gbps"], 3),
+            "batched_vs_plain": round(rm["batched"]["gbps"] / rm["plain"]["gbps"], 3),
+            "target": ">= 3x plain", "ratio": round(rm["ratio"], 5), "autotune": rm["tune"],
+            "k_match_find_ms": round(rm["k1_ms"], 3), "pipeline_ms": round(rm["pipeline_ms"], 3),
+            "k_match_find_frac_hbm": round((n_in + n_out) / (rm["k1_ms"] / 1e3) / 1e9 / HBM_PEAK_GBPS, 5),
+            "check": "plain == batched chunk by chunk; whole output decodes; first 2 chunks == oracle"}
 
 
 def main():
@@ -394,16 +437,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mixed", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="leave out the mixed-size comparison")
     ap.add_argument("--corpus", nargs="+", default=[], metavar="KIND:MIB",
                     help="also run the plain and batched forms on KIND (text, binmix, log) of MIB MiB in "
-                         "2 MiB chunks, e.g. text:16 log:1024")
-    ap.add_argument("--serve", action="store_true")
+                         "2 MiB chunks, e.g. text:16 log:1024 (extra JSON lines)")
+    ap.add_argument("--no-serve", action="store_true", help="leave out the serving form")
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-submitters", type=int, default=4, help="threads sharing the arrival schedule")
-    ap.add_argument("--serve-load", type=float, nargs="+", default=[2.0, 4.0, 8.0, 12.0],
-                    help="offered loads (GB/s) of the serving runs, one run each")
+    ap.add_argument("--serve-load", type=float, nargs="+", default=[8.0],
+                    help="offered loads (GB/s) of the serving runs; the first goes into the headline line, "
+                         "the others are extra JSON lines")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true", help="leave out the host-buffer end-to-end rates")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy (y = x + 1, uint8) of MIB MiB three times; "
                          "profiling its FETCH_SIZE / WRITE_SIZE gives the counter-to-bytes factor")
@@ -464,19 +509,42 @@ def main():
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
                      "traffic": pc.get("traffic_bytes_per_launch"), "alg_bytes_per_launch": alg_bytes,
                      "valu_issue_frac": pc.get("valu_issue_frac"), "valu_lane_frac": pc.get("valu_lane_frac"),
-                     "traffic_source": "profiles/k_match_find_binmix256_traffic.md (rocprofv3 --pmc)",
+                     "traffic_source": ("profiles/k_match_find_binmix256_traffic.md (rocprofv3 --pmc, "
+                                        "same kernel sources)") if pc else
+                                       "null: the committed counter profile is of other kernel sources",
                      "avg_launch_ms": round(r["k1_ms"], 4), "seg_table_ms": round(r["k0_ms"], 4),
                      "emit_ms": round(r["emit_ms"], 4),
                      "pipeline_ms": round(r["pipeline_ms"], 4)},
     }
-    log("end to end (host buffers) ...")
-    line["end_to_end"] = end_to_end(ctx, data, sizes, r["chunks_out"])
+    if not args.no_e2e:
+        log("end to end (host buffers) ...")
+        line["end_to_end"] = end_to_end(ctx, data, sizes, r["chunks_out"])
     if not args.no_cpu:
-        log("cpu baselines ...")
-        port, zl = cpu_baselines(data, 64 * MiB, 2 * MiB)
-        line["cpu_baseline"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in port.items()}
-        line["cpu_zlib"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in zl.items()}
+        log("cpu baseline ...")
+        cb = cpu_baseline(data, 2 * MiB)
+        line["cpu_baseline"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in cb.items()}
+        port = cpu_port(data, 64 * MiB, 2 * MiB)
+        line["cpu_port"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in port.items()}
+    del r
+    if not args.no_mixed:
+        log("mixed sizes: 4096 chunks, 4 KiB - 4 MiB ...")
+        line["mixed"] = mixed_sizes(torch, ctx, max(3, args.steps // 2))
+        torch.cuda.empty_cache()
+    extra = []
+    if not args.no_serve:
+        for i, load in enumerate(args.serve_load):
+            log(f"serving: {args.serve_requests} requests at {load} GB/s offered ...")
+            rs = serve(ctx, args.serve_requests, load, submitters=args.serve_submitters)
+            rs = {"workload": f"{args.serve_requests} requests, Poisson arrivals at {load} GB/s offered, sizes "
+                              "log-normal (median 256 KiB, sigma 2.0, 1 KiB-64 MiB), content 1/3 each text / "
+                              "binmix / log, from host buffers (PCIe both ways), 64 GB device arena", **rs}
+            if i == 0:
+                line["serving"] = rs
+            else:
+                extra.append(rs)
     print(json.dumps(line), flush=True)
+    for rs in extra:
+        print(json.dumps(rs), flush=True)
 
     for spec in args.corpus:
         kind, mib = spec.split(":")
@@ -488,23 +556,6 @@ def main():
         print(json.dumps({"workload": f"{kind} {mib} MiB, 2 MiB chunks", "input_bytes": rc["n_in"],
                           "plain_gbps": round(rc["plain"]["gbps"], 3), "batched_gbps": round(rc["batched"]["gbps"], 3),
                           "ratio": round(rc["ratio"], 5), "autotune": rc["tune"]}), flush=True)
-    if args.mixed:
-        log("mixed sizes: 4096 chunks, 4 KiB - 4 MiB ...")
-        mdata, msizes = mixed_sizes_workload()
-        rm = plain_and_batched(torch, ctx, mdata, msizes, max(3, args.steps // 2), 1)
-        print(json.dumps({"workload": "mixed sizes, 4096 chunks log-uniform 4 KiB-4 MiB, text/binmix/log 1/3 each",
-                          "input_bytes": rm["n_in"], "plain_gbps": round(rm["plain"]["gbps"], 3),
-                          "batched_gbps": round(rm["batched"]["gbps"], 3),
-                          "batched_vs_plain": round(rm["batched"]["gbps"] / rm["plain"]["gbps"], 3),
-                          "ratio": round(rm["ratio"], 5),
-                          "k1_frac_hbm": round((rm["n_in"] + rm["n_out"]) / (rm["k1_ms"] / 1e3) / 1e9 / HBM_PEAK_GBPS, 5)}),
-              flush=True)
-    if args.serve:
-        for load in args.serve_load:
-            log(f"serving: {args.serve_requests} requests at {load} GB/s offered ...")
-            rs = serve(torch, ctx, args.serve_requests, load, submitters=args.serve_submitters)
-            print(json.dumps({"workload": "serving, Poisson arrivals, log-normal sizes (median 256 KiB, sigma 2), "
-                                          "host buffers (PCIe both ways)", **rs}), flush=True)
     ctx.close()
 
 

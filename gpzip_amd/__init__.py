@@ -313,10 +313,11 @@ class Server:
         _check(self.L.gpz_request_release(self.h, tag.value))
         return tag.value, data, lat.value
 
-    def stats(self):
-        v = (C.c_uint64 * 4)()
-        _check(self.L.gpz_server_stats(self.h, v, 4))
-        return list(v)
+    def stats(self) -> dict:
+        v = (C.c_uint64 * 7)()
+        _check(self.L.gpz_server_stats(self.h, v, 7))
+        keys = ["done", "bytes_done", "steps", "peak_bytes", "arena_bytes", "waited", "workspace_bytes"]
+        return dict(zip(keys, list(v)))
 
     def stop(self) -> None:
         if self.h:

@@ -93,30 +93,36 @@ static int capture_graph(const Work &w, const Grids &g, hipStream_t st, hipGraph
     return GPZ_OK;
 }
 
+int lane_step_resident(gpz_ctx *ctx, Lane *ln, const Plan &pl, const void *d_in) {
+    int rc;
+    if ((rc = ln->wsp.reserve(Caps{pl.P, pl.G, pl.M, pl.C, pl.out_cap}))) return rc;
+    HIPCHK(hipSetDevice(ctx_device(ctx)));
+    if ((rc = ln->wsp.upload_plan(pl, ln->st))) return rc;
+    Work w = ln->wsp.work;
+    w.in = (const uint8_t *)d_in;
+    w.gzip = 1;
+    w.strict = 0;
+    Grids g = grids_for(ln->wsp.caps, ctx_cus(ctx));
+    apply_tune(g, ctx_tune(ctx), ln->wsp.caps.G, ln->wsp.caps.M);  // capacity: stable, so the graph is kept
+    if (!ln->graph || ln->graph_gen != ln->wsp.generation || ln->graph_in != d_in || !(ln->graph_grids == g)) {
+        if ((rc = capture_graph(w, g, ln->st, &ln->graph))) return rc;
+        ln->graph_gen = ln->wsp.generation;
+        ln->graph_in = d_in;
+        ln->graph_grids = g;
+    }
+    HIPCHK(hipGraphLaunch(ln->graph, ln->st));
+    return GPZ_OK;
+}
+
 int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *const *ptrs, const uint64_t *lens,
                       unsigned copy_threads) {
     int rc;
-    if ((rc = ln->wsp.reserve(Caps{pl.P, pl.G, pl.M, pl.C, pl.out_cap}))) return rc;
     if ((rc = ln->din.ensure(pl.in_bytes))) return rc;
     if ((rc = ln->hin.ensure(pl.in_bytes))) return rc;
     HIPCHK(hipSetDevice(ctx_device(ctx)));
     copy_pieces((uint8_t *)ln->hin.p, pl.chunk_dev_off.data(), ptrs, lens, pl.C, copy_threads);
     HIPCHK(hipMemcpyAsync(ln->din.p, ln->hin.p, pl.in_end, hipMemcpyHostToDevice, ln->st));
-    if ((rc = ln->wsp.upload_plan(pl, ln->st))) return rc;
-    Work w = ln->wsp.work;
-    w.in = (const uint8_t *)ln->din.p;
-    w.gzip = 1;
-    w.strict = 0;
-    Grids g = grids_for(ln->wsp.caps, ctx_cus(ctx));
-    apply_tune(g, ctx_tune(ctx), ln->wsp.caps.G, ln->wsp.caps.M);  // capacity: stable, so the graph is kept
-    if (!ln->graph || ln->graph_gen != ln->wsp.generation || ln->graph_in != ln->din.p || !(ln->graph_grids == g)) {
-        if ((rc = capture_graph(w, g, ln->st, &ln->graph))) return rc;
-        ln->graph_gen = ln->wsp.generation;
-        ln->graph_in = ln->din.p;
-        ln->graph_grids = g;
-    }
-    HIPCHK(hipGraphLaunch(ln->graph, ln->st));
-    return GPZ_OK;
+    return lane_step_resident(ctx, ln, pl, ln->din.p);
 }
 
 int lane_batch_collect_pinned(Lane *ln, const Plan &pl, const uint8_t **data, std::vector<uint64_t> &off,

@@ -1141,23 +1141,42 @@ __global__ __launch_bounds__(1024) void k_hist_tokens(Work w) {
 // k_huffman: one wave per member
 
 struct HuffScratch {
-    __align__(16) uint32_t ck[292];  // used leaves' keys (freq << 10 | id), then 4 x ~0
-    uint32_t sl[288];                 // the same keys sorted
-    uint32_t ik[288];                 // internal nodes' keys (freq << 10 | 1 << 9 | index)
+    // keys of the used leaves, the sorted leaves and the internal nodes: 32- or
+    // 64-bit (build_lengths<K>), so sized for 64
+    __align__(16) uint64_t ck[292];  // used leaves' keys (freq << 10 | id), then 4 x ~0
+    uint64_t sl[288];                 // the same keys sorted
+    uint64_t ik[288];                 // internal nodes' keys (freq << 10 | 1 << 9 | index)
     uint16_t hg[288];                 // internal nodes' heights
     uint16_t par[576];                // parent: leaf id -> par[id], internal j -> par[288 + j]
     uint16_t rb[290];                 // first internal node of each merge round
     uint8_t dep[288];                 // internal nodes' depths
 };
 
+__device__ __forceinline__ uint32_t uni_k(uint32_t v) { return uni(v); }
+__device__ __forceinline__ uint64_t uni_k(uint64_t v) { return uni64(v); }
+__device__ __forceinline__ uint32_t lane1_k(uint32_t v) { return __builtin_amdgcn_readlane(v, 1); }
+__device__ __forceinline__ uint64_t lane1_k(uint64_t v) {
+    return ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), 1) << 32) |
+           __builtin_amdgcn_readlane((uint32_t)v, 1);
+}
+__device__ __forceinline__ uint32_t shfl_k(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, 64); }
+__device__ __forceinline__ uint64_t shfl_k(uint64_t v, int src) {
+    return ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+}
+__device__ __forceinline__ uint32_t shfl_xor_k(uint32_t v, int d) { return (uint32_t)__shfl_xor((int)v, d, 64); }
+__device__ __forceinline__ uint64_t shfl_xor_k(uint64_t v, int d) {
+    return ((uint64_t)(uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), d, 64) << 32) |
+           (uint32_t)__shfl_xor((int)(uint32_t)v, d, 64);
+}
+
 // try_build_code_lengths (huffman.rs:26-60) for n <= 288 symbols on one
 // wave.  The reference pops the minimum (freq, id) from a binary heap whose
 // internal nodes get ids n, n+1, ...; merged frequencies never decrease and
 // internal ids exceed every leaf id, so every pop takes the least item under
 // the order (freq, leaf before internal, leaf id / creation index).  Items
-// are 32-bit keys freq << 10 | internal << 9 | index in that order (totals
-// stay below 2^22: a histogram counts at most one token per input byte of a
-// 128 KiB member).
+// are keys freq << 10 | internal << 9 | index in that order, of type K: 32
+// bits while every frequency sum stays below 2^22 (a member of fewer than
+// 2^22 - 1 tokens), 64 bits otherwise.
 //
 // Merge rounds: the 64 least remaining items are the least 64 of the 64
 // least leaves and the 64 least queued internal nodes, sorted by one bitonic
@@ -1169,10 +1188,14 @@ struct HuffScratch {
 // made in a later round, so depths follow from the rounds in reverse.
 // Frequencies are halved while the longest code exceeds `limit`
 // (huffman.rs:44-58).
+template <typename K>
 __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens, HuffScratch &hs) {
     const uint32_t lane = threadIdx.x;
     const uint64_t lt = (1ull << lane) - 1;
-    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    constexpr K NONE = ~(K)0;
+    K *ck = reinterpret_cast<K *>(hs.ck);
+    K *sl = reinterpret_cast<K *>(hs.sl);
+    K *ik = reinterpret_cast<K *>(hs.ik);
     uint32_t fr[5];
 #pragma unroll
     for (int j = 0; j < 5; j++) fr[j] = (int)(lane + 64 * j) < n ? f[lane + 64 * j] : 0u;
@@ -1182,7 +1205,7 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
 #pragma unroll
         for (int j = 0; j < 5; j++) {
             const uint64_t m = __ballot(fr[j] != 0);
-            if (fr[j]) hs.ck[u + __popcll(m & lt)] = (fr[j] << 10) | (lane + 64 * j);
+            if (fr[j]) ck[u + __popcll(m & lt)] = ((K)fr[j] << 10) | (lane + 64 * j);
             u += __popcll(m);
         }
         u = uni(u);
@@ -1193,24 +1216,25 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
             __syncthreads();
             return;
         }
-        if (lane < 4) hs.ck[u + lane] = NONE;
+        if (lane < 4) ck[u + lane] = NONE;
         __syncthreads();
         // sorted by rank (keys are distinct)
         {
-            uint32_t kk[5], r[5];
+            K kk[5];
+            uint32_t r[5];
 #pragma unroll
             for (int j = 0; j < 5; j++) {
-                kk[j] = lane + 64 * j < u ? hs.ck[lane + 64 * j] : NONE;
+                kk[j] = lane + 64 * j < u ? ck[lane + 64 * j] : NONE;
                 r[j] = 0;
             }
             for (uint32_t i = 0; i < u; i += 4) {
-                const uint4 k4 = *reinterpret_cast<const uint4 *>(&hs.ck[i]);
+                const K a0 = ck[i], a1 = ck[i + 1], a2 = ck[i + 2], a3 = ck[i + 3];
 #pragma unroll
-                for (int j = 0; j < 5; j++) r[j] += (k4.x < kk[j]) + (k4.y < kk[j]) + (k4.z < kk[j]) + (k4.w < kk[j]);
+                for (int j = 0; j < 5; j++) r[j] += (a0 < kk[j]) + (a1 < kk[j]) + (a2 < kk[j]) + (a3 < kk[j]);
             }
 #pragma unroll
             for (int j = 0; j < 5; j++)
-                if (kk[j] != NONE) hs.sl[r[j]] = kk[j];
+                if (kk[j] != NONE) sl[r[j]] = kk[j];
         }
         __syncthreads();
         // merge rounds
@@ -1219,28 +1243,29 @@ __device__ void build_lengths(const uint32_t *f, int n, int limit, uint8_t *lens
             // leaves ascending, queued internal nodes descending: one bitonic
             // sequence of 128 whose lower half after the first exchange holds
             // the 64 least; half-cleaners sort it
-            const uint32_t a = nl + lane < u ? hs.sl[nl + lane] : NONE;
+            const K a = nl + lane < u ? sl[nl + lane] : NONE;
             const uint32_t bi = qh + 63 - lane;
-            const uint32_t b = bi < k ? hs.ik[bi] : NONE;
-            uint32_t v = min(a, b);
+            const K b = bi < k ? ik[bi] : NONE;
+            K v = a < b ? a : b;
 #pragma unroll
             for (int d = 32; d >= 1; d >>= 1) {
-                const uint32_t o = (uint32_t)__shfl_xor((int)v, d, 64);
-                v = (lane & d) ? max(v, o) : min(v, o);
+                const K o = shfl_xor_k(v, d);
+                v = (lane & d) ? (v > o ? v : o) : (v < o ? v : o);
             }
-            const uint32_t thr = (uni(v) >> 10) + (__builtin_amdgcn_readlane(v, 1) >> 10);
+            const K thr = (uni_k(v) >> 10) + (lane1_k(v) >> 10);
             const uint32_t T = __popcll(__ballot((v >> 10) <= thr)) >> 1;  // >= 1
-            const uint32_t x = (uint32_t)__shfl((int)v, (int)(2 * lane) & 63, 64);
-            const uint32_t y = (uint32_t)__shfl((int)v, (int)(2 * lane + 1) & 63, 64);
+            const K x = shfl_k(v, (int)(2 * lane) & 63);
+            const K y = shfl_k(v, (int)(2 * lane + 1) & 63);
             if (lane == 0) hs.rb[nr] = (uint16_t)k;
             if (lane < T) {
                 const uint32_t node = k + lane;
                 const bool xi = (x >> 9) & 1u, yi = (y >> 9) & 1u;
-                const uint32_t hx = xi ? hs.hg[x & 0x1FFu] : 0u, hy = yi ? hs.hg[y & 0x1FFu] : 0u;
-                hs.ik[node] = (((x >> 10) + (y >> 10)) << 10) | 0x200u | node;
+                const uint32_t xs = (uint32_t)x & 0x1FFu, ys = (uint32_t)y & 0x1FFu;
+                const uint32_t hx = xi ? hs.hg[xs] : 0u, hy = yi ? hs.hg[ys] : 0u;
+                ik[node] = (((x >> 10) + (y >> 10)) << 10) | 0x200u | node;
                 hs.hg[node] = (uint16_t)(max(hx, hy) + 1);
-                hs.par[(xi ? 288u : 0u) + (x & 0x1FFu)] = (uint16_t)node;
-                hs.par[(yi ? 288u : 0u) + (y & 0x1FFu)] = (uint16_t)node;
+                hs.par[(xi ? 288u : 0u) + xs] = (uint16_t)node;
+                hs.par[(yi ? 288u : 0u) + ys] = (uint16_t)node;
             }
             const uint32_t lu = __popcll(__ballot(lane < 2 * T && !((v >> 9) & 1u)));
             nl += lu;
@@ -1384,8 +1409,15 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
         __syncthreads();
         if (lane == 0) freq[256] += 1;  // end of block
         __syncthreads();
-        build_lengths(freq, 286, 15, lens, hs);
-        build_lengths(freq + DSYM0, 30, 15, lens + DSYM0, hs);
+        // 32-bit heap keys hold frequency sums below 2^22: members of fewer
+        // than 2^22 - 1 tokens (every member of the 128 KiB split)
+        if (md.n + 1u < (1u << 22)) {
+            build_lengths<uint32_t>(freq, 286, 15, lens, hs);
+            build_lengths<uint32_t>(freq + DSYM0, 30, 15, lens + DSYM0, hs);
+        } else {
+            build_lengths<uint64_t>(freq, 286, 15, lens, hs);
+            build_lengths<uint64_t>(freq + DSYM0, 30, 15, lens + DSYM0, hs);
+        }
         {
             // single/zero distance-code fix-up and litlen fix-up (deflate.rs:855-867)
             const uint64_t dm = __ballot(lane < 30 && lens[DSYM0 + lane] != 0);
@@ -1469,7 +1501,7 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
         __syncthreads();
         for (uint32_t r = lane; r < K; r += 64) atomicAdd(&mfreq[rle[r] & 0xFF], 1u);
         __syncthreads();
-        build_lengths(mfreq, 19, 7, mlens, hs);
+        build_lengths<uint32_t>(mfreq, 19, 7, mlens, hs);
         canonical_rev(mlens, 19, mcodes);
         // dynamic block header (deflate.rs:886-930): all fields at known
         // offsets, the RLE codes placed by a scan of their bit counts

@@ -82,15 +82,23 @@ struct HostBuf {
     ~HostBuf();
 };
 
+Layout layout_for(const Caps &c);
+
 struct Workspace {
     DevBuf ws;
     HostBuf hdesc;
     Caps caps;
     Layout lay{};
     Work work{};
+    uint8_t *base = nullptr;  // device memory the layout starts at
+    bool external = false;    // laid out in memory it does not own (bind): fixed capacity
     uint64_t generation = 0;
+    // Grow (own memory) to hold `want`.
     int reserve(const Caps &want);
+    // Lay out capacity c in caller-owned device memory of `bytes` bytes.
+    int bind(const Caps &c, void *mem, uint64_t bytes);
     int upload_plan(const Plan &pl, hipStream_t st);
+    void point(uint8_t *b);
 };
 
 struct Lane {
@@ -121,6 +129,9 @@ int lane_fetch_chunk(Lane *ln, uint32_t c, uint8_t *out, size_t out_cap, size_t 
 // the lane's graph.  Returns once enqueued.
 int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *const *ptrs, const uint64_t *lens,
                       unsigned copy_threads);
+// Batched step over chunks already in device memory (plan offsets relative to
+// d_in): upload the plan, replay the lane's graph.  Returns once enqueued.
+int lane_step_resident(gpz_ctx *ctx, Lane *ln, const Plan &pl, const void *d_in);
 // Wait for the step and copy every chunk's members to host.
 int lane_batch_collect(Lane *ln, const Plan &pl, std::unique_ptr<uint8_t[]> &out, std::vector<uint64_t> &off,
                        std::vector<uint64_t> &len);

@@ -155,15 +155,7 @@ static uint64_t carve(uint64_t &cur, uint64_t bytes) {
     return o;
 }
 
-int Workspace::reserve(const Caps &want) {
-    Caps c = caps;
-    bool grow = !ws.p || want.P > c.P || want.G > c.G || want.M > c.M || want.C > c.C || want.out > c.out;
-    if (!grow) return GPZ_OK;
-    c.P = std::max(c.P, want.P);
-    c.G = std::max(c.G, want.G);
-    c.M = std::max(c.M, want.M);
-    c.C = std::max(c.C, want.C);
-    c.out = std::max(c.out, want.out);
+Layout layout_for(const Caps &c) {
     uint64_t cur = 0;
     Layout L;
     L.mem = carve(cur, c.M * sizeof(MemberDesc));
@@ -192,14 +184,12 @@ int Workspace::reserve(const Caps &want) {
     L.total = carve(cur, 8);
     L.out = carve(cur, c.out);
     L.end = cur;
-    int rc = ws.ensure(L.end);
-    if (rc) return rc;
-    rc = hdesc.ensure(L.desc_end);
-    if (rc) return rc;
-    caps = c;
-    lay = L;
-    generation++;
-    uint8_t *b = (uint8_t *)ws.p;
+    return L;
+}
+
+// Point the kernel arguments at a workspace laid out by `lay` from b.
+void Workspace::point(uint8_t *b) {
+    const Layout &L = lay;
     Work &w = work;
     w = Work{};
     w.mem = (const MemberDesc *)(b + L.mem);
@@ -227,6 +217,41 @@ int Workspace::reserve(const Caps &want) {
     w.total = (uint64_t *)(b + L.total);
     w.out = b + L.out;
     w.window = 32768;
+    base = b;
+}
+
+int Workspace::reserve(const Caps &want) {
+    Caps c = caps;
+    bool grow = !base || want.P > c.P || want.G > c.G || want.M > c.M || want.C > c.C || want.out > c.out;
+    if (!grow) return GPZ_OK;
+    if (external) return fail(GPZ_E_INVAL, "step exceeds the fixed workspace of a serving lane");
+    c.P = std::max(c.P, want.P);
+    c.G = std::max(c.G, want.G);
+    c.M = std::max(c.M, want.M);
+    c.C = std::max(c.C, want.C);
+    c.out = std::max(c.out, want.out);
+    const Layout L = layout_for(c);
+    int rc = ws.ensure(L.end);
+    if (rc) return rc;
+    rc = hdesc.ensure(L.desc_end);
+    if (rc) return rc;
+    caps = c;
+    lay = L;
+    generation++;
+    point((uint8_t *)ws.p);
+    return GPZ_OK;
+}
+
+int Workspace::bind(const Caps &c, void *mem, uint64_t bytes) {
+    const Layout L = layout_for(c);
+    if (L.end > bytes) return fail(GPZ_E_INVAL, "workspace needs %llu bytes", (unsigned long long)L.end);
+    int rc = hdesc.ensure(L.desc_end);
+    if (rc) return rc;
+    caps = c;
+    lay = L;
+    external = true;
+    generation++;
+    point((uint8_t *)mem);
     return GPZ_OK;
 }
 
@@ -238,7 +263,7 @@ int Workspace::upload_plan(const Plan &pl, hipStream_t st) {
     memcpy(h + lay.chunk_mem0, pl.chunk_mem0.data(), pl.chunk_mem0.size() * 4);
     Counts cn{pl.G, pl.M, pl.C, 0};
     memcpy(h + lay.counts, &cn, sizeof(cn));
-    HIPCHK(hipMemcpyAsync(ws.p, hdesc.p, lay.desc_end, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base, hdesc.p, lay.desc_end, hipMemcpyHostToDevice, st));
     return GPZ_OK;
 }
 
@@ -694,6 +719,10 @@ int gpz_encode_block(gpz_ctx *ctx, const uint32_t *walked, size_t n, uint8_t *ou
     uint64_t l = n;
     int rc = build_plan(pl, &l, nullptr, 1, (uint32_t)std::max<size_t>(n, 1), 0);
     if (rc) return rc;
+    // build_plan bounds byte inputs (2 B per byte); a token costs up to 48 bits
+    // (15-bit length code + 5 extra + 15-bit distance code + 13 extra), and the
+    // block header at most PREFIX_WORDS words
+    pl.out_cap = round_up(6 * (uint64_t)n + PREFIX_WORDS * 4 + 64, 256) + 256;
     Lane *ln = ctx->acquire();
     do {
         Caps want{pl.P, pl.G, pl.M, pl.C, pl.out_cap};

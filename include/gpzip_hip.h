@@ -200,18 +200,26 @@ void gpz_plan_destroy(gpz_plan *p);
 
 typedef struct gpz_server gpz_server;
 
+/* Device memory of a server is one arena of budget_bytes, allocated by
+ * gpz_server_start: `streams` lane workspaces of fixed capacity (hash tables,
+ * tokens, Huffman tables, step output) and an input pool.  Admitted requests
+ * wait in the pool, in HBM, until compressed. */
 typedef struct gpz_server_config {
-    uint64_t budget_bytes;     /* device-memory budget for admitted work */
+    uint64_t budget_bytes;     /* device memory of the arena (workspaces + input pool) */
     uint32_t max_batch_chunks; /* chunks gathered into one device step */
     uint32_t max_batch_bytes;  /* input bytes gathered into one device step (8 MiB) */
-    uint32_t chunk_bytes;      /* request split size (2 MiB) */
-    uint32_t streams;          /* device steps in flight */
+    uint32_t chunk_bytes;      /* request split size (2 MiB; a multiple of 256) */
+    uint32_t streams;          /* device steps in flight (worker lanes, at most 16) */
 } gpz_server_config;
 
 void gpz_server_config_default(gpz_server_config *cfg);
+/* GPZ_E_INVAL if budget_bytes cannot hold the lane workspaces plus one chunk
+ * (gpz_server_stats [6] of a started server gives the workspace bytes). */
 int gpz_server_start(gpz_ctx *ctx, const gpz_server_config *cfg, gpz_server **out);
-/* Admit a request (the bytes are copied).  GPZ_E_BUSY if it can never fit
- * the budget; otherwise blocks until admitted. */
+/* Admit a request: its bytes are copied into the arena's input pool before
+ * return.  GPZ_E_BUSY if it can never fit the pool; otherwise blocks until it
+ * is admitted (first come, first served).  A tag may not be in use (submitted
+ * and not yet released). */
 int gpz_request_submit(gpz_server *s, const uint8_t *in, size_t len, uint64_t tag);
 /* Next finished request, in completion order; each request's members are
  * concatenated in input order.  timeout_ms < 0 waits forever.  The buffer is
@@ -222,8 +230,11 @@ int gpz_request_next(gpz_server *s, int timeout_ms, uint64_t *tag, const uint8_t
  * not returned yet (the server may still be writing that request's output). */
 int gpz_request_release(gpz_server *s, uint64_t tag);
 /* Counters: [0] requests done, [1] input bytes done, [2] device steps,
- * [3] peak admitted bytes. */
+ * [3] peak arena bytes in use (workspaces + admitted inputs), [4] arena bytes,
+ * [5] admissions that had to wait for pool space, [6] lane workspace bytes. */
 int gpz_server_stats(gpz_server *s, uint64_t *out, size_t n);
+/* Stops after every admitted request is compressed.  No other call on s may
+ * overlap it. */
 void gpz_server_stop(gpz_server *s);
 
 #ifdef __cplusplus

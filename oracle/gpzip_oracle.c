@@ -13,14 +13,24 @@
  *   orc_match_find       - against orc_match_find_brute (independent,
  *                          table-free statement of the candidate rule),
  *                          hand-derived known answers (q x 64, abc x 8,
- *                          < 3 bytes, 32 distinct bytes) and invariants
+ *                          < 3 bytes, 32 distinct bytes, the candidate order
+ *                          and window cases, and hash3 collisions that
+ *                          shadow a real candidate) and invariants
  *                          (every match verifies, 3<=len<=258, dist<=32768,
  *                          reconstruct(walk(tok)) == input).
+ *   orc_hash3            - the collision known answers above: a build with
+ *                          another multiplier (ORC_HASH_MUL) or byte order
+ *                          fails them (tests/test_oracle.py builds such a
+ *                          mutant and checks that it does).
  *   orc_greedy_walk      - closed-form cases + the round trip.
  *   orc_strict_walk      - the reference's exact-equality walk cases
  *                          (walk.rs:575-695) as fixtures.
  *   orc_build_code_lengths - reference property tests (huffman.rs:158-232),
- *                          brute-force optimality on small alphabets, Kraft.
+ *                          brute-force optimality on small alphabets, Kraft,
+ *                          hand-derived halving cases (limits 7 and 4), and
+ *                          an independent heap model on random vectors; a
+ *                          build that rounds the halving up (ORC_HALVE) fails
+ *                          the halving cases.
  *   orc_encode_block_fast - zlib inflate (independent decoder), the
  *                          hand-derived empty-stream KAT, gzip -t.
  *   orc_crc32            - zlib.crc32.
@@ -31,10 +41,19 @@
 #include <string.h>
 
 /* ------------------------------------------------------------------ */
-/* hash3 -- lz77_hash_build.wgsl:36-43 (byte order: lz77_hash_build.wgsl:30-34) */
+/* hash3 -- lz77_hash_build.wgsl:36-43 (byte order: lz77_hash_build.wgsl:30-34).
+ * ORC_HASH_MUL and ORC_HASH_REVERSED exist so that a test can build mutants
+ * and show that the known answers reject them. */
+#ifndef ORC_HASH_MUL
+#define ORC_HASH_MUL 0x9E3779B1u
+#endif
 uint32_t orc_hash3(const uint8_t *in, size_t p) {
+#ifdef ORC_HASH_REVERSED /* mutant: bytes in the other order */
+    uint32_t x = ((uint32_t)in[p + 2] << 16) | ((uint32_t)in[p + 1] << 8) | (uint32_t)in[p];
+#else
     uint32_t x = ((uint32_t)in[p] << 16) | ((uint32_t)in[p + 1] << 8) | (uint32_t)in[p + 2];
-    return (uint32_t)(x * 0x9E3779B1u) >> (32u - ORC_HASH_BITS);
+#endif
+    return (uint32_t)(x * (uint32_t)ORC_HASH_MUL) >> (32u - ORC_HASH_BITS);
 }
 
 /* try_candidate -- lz77_hash_lookup.wgsl:45-67.  Returns the accepted length
@@ -315,7 +334,12 @@ static void standard_huffman(const uint32_t *freq, size_t n, uint8_t *lens) {
     free(depth);
 }
 
-/* try_build_code_lengths -- huffman.rs:26-60 */
+/* try_build_code_lengths -- huffman.rs:26-60.  ORC_HALVE is the halving
+ * step (huffman.rs:52-56, rounding down); a test builds a mutant that rounds
+ * up and shows that the known answers reject it. */
+#ifndef ORC_HALVE
+#define ORC_HALVE(f) ((f) / 2u)
+#endif
 int orc_build_code_lengths(const uint32_t *freq, size_t n, uint32_t max_bits, uint8_t *lens) {
     size_t used = 0, only = 0;
     for (size_t i = 0; i < n; i++)
@@ -341,7 +365,7 @@ int orc_build_code_lengths(const uint32_t *freq, size_t n, uint32_t max_bits, ui
             if (lens[i] > mx) mx = lens[i];
         if (mx <= max_bits) break;
         for (size_t i = 0; i < n; i++)
-            if (work[i] > 1) work[i] /= 2;
+            if (work[i] > 1) work[i] = ORC_HALVE(work[i]);
     }
     free(work);
     return 1;

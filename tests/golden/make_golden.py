@@ -9,7 +9,11 @@ wgpu).  The vectors are
     whose expected output the reference defines as host_strict_greedy, and
     hand-written lazy-walk cases (codec-gpu/src/gpu/lz77.rs:260-284);
   * Huffman cases from codec-gpu/src/gpu/huffman.rs:158-232 plus a tie-break
-    case;
+    case and length-limit halving cases (two derived by hand, one from the
+    heap statement in tests/_vectors.py);
+  * symbol-frequency vectors of token streams whose litlen, distance and
+    code-length codes exceed their limits (15, 15, 7), so that the halving
+    loop runs in each;
   * the empty-token-stream DEFLATE block derived by hand and checked with zlib.
 
 Run:  python tests/golden/make_golden.py
@@ -20,8 +24,11 @@ import base64
 import json
 import os
 import random
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from _vectors import heap_code_lengths  # noqa: E402
 
 
 def hash3(b: bytes, p: int) -> int:
@@ -129,7 +136,42 @@ def match_kats():
     body = (b"vwxyz" * 200)[:600]
     buf = construct(1000, {100: body}, [110], 9, allow_internal=True)
     kats.append({"name": "cap_258", "input": buf, "expect": {110: [258, 10]}})
+    # I: hash collisions shadow the real candidate.  "EHN" and "_gk" hash like
+    # "ABC" under lz77_hash_build.wgsl:36-43 (checked below).  Segment 0 holds
+    # EHN at 10 (its oldest entry for that hash), the real ABCD at 500 and _gk
+    # at 1000 (its newest).  The query at 4200 (segment 1) is the only such
+    # hash in its own segment, so it rejects itself (cand >= p); in segment 0
+    # newest (_gk) and then oldest (EHN, since best is 0 < 16) fail the
+    # three-byte check, and the real occurrence between them is never tried:
+    # a literal.  A wrong multiplier or byte order makes EHN and _gk hash
+    # elsewhere, so 500 becomes both oldest and newest and gives (4, 3700).
+    assert COLLIDERS_OF_ABC == [b"EHN", b"_gk"]
+    plants = {10: b"EHN1", 500: b"ABCD", 1000: b"_gk2", 4200: b"ABCDQ"}
+    buf = construct(8192, plants, [4200], 10)
+    kats.append({"name": "hash3_collisions_shadow", "input": buf, "expect": {4200: lit(ord("A"))}})
+    # J: the colliding 3-gram is the newest entry and fails the three-byte
+    # check; best stays 0 < 16, so the oldest entry (the real ABCDE at 10) is
+    # tried: (5, 4190).
+    plants = {10: b"ABCDE", 1000: b"_gk3", 4200: b"ABCDEx"}
+    buf = construct(8192, plants, [4200], 11)
+    kats.append({"name": "hash3_collision_newest_then_oldest", "input": buf, "expect": {4200: [5, 4190]}})
     return kats
+
+
+def _colliders(target: bytes, alphabet: range) -> list[bytes]:
+    h = hash3(target, 0)
+    out = []
+    for b0 in alphabet:
+        for b1 in alphabet:
+            for b2 in alphabet:
+                g = bytes([b0, b1, b2])
+                if g != target and hash3(g, 0) == h:
+                    out.append(g)
+    return out
+
+
+# printable 3-grams (0x41..0x7a) that collide with "ABC": the first two, sorted
+COLLIDERS_OF_ABC = sorted(_colliders(b"ABC", range(0x41, 0x7B)))[:2]
 
 
 def strict_walk(tokens):
@@ -203,7 +245,92 @@ def huffman_cases():
         {"name": "tie_leaf_first", "freq": [1, 1, 2, 2], "limit": 15, "expect": [2, 2, 2, 2]},
         # (1,0)+(2,1) -> n3:3, (3,2)+(3,n3) -> root: sym2 depth 1, syms 0,1 depth 2
         {"name": "three", "freq": [1, 2, 3], "limit": 15, "expect": [2, 2, 1]},
+        # Length-limit halving (huffman.rs:44-58: every f > 1 becomes f / 2,
+        # rounded down), derived by hand.  Fibonacci frequencies give the chain
+        # tree [8,8,7,6,5,4,3,2,1]: 8 > 7, so halve to [1,1,1,1,2,4,6,10,17].
+        # Heap pops: (1,0)+(1,1)=n9:2, (1,2)+(1,3)=n10:2, (2,4)+(2,n9)=n11:4,
+        # (2,n10)+(4,5)=n12:6, (4,n11)+(6,6)=n13:10, (6,n12)+(10,7)=n14:16,
+        # (10,n13)+(16,n14)=n15:26, (17,8)+(26,n15)=root.  Depths: sym8 1;
+        # n13, n14 2; syms 6, 7 and n11, n12 3; syms 4, 5 and n9, n10 4;
+        # syms 0-3 5.  Rounding up instead ((f+1)/2 -> [1,1,1,2,3,4,7,11,17])
+        # gives [5,5,4,4,4,3,3,2,2].
+        {"name": "halve_once_limit7", "freq": [1, 1, 2, 3, 5, 8, 13, 21, 34], "limit": 7,
+         "expect": [5, 5, 5, 5, 4, 4, 3, 3, 1]},
+        # The same at limit 4: [5,5,5,5,4,4,3,3,1] still exceeds 4, so halve
+        # again to [1,1,1,1,1,2,3,5,8]: (1,0)+(1,1)=n9:2, (1,2)+(1,3)=n10:2,
+        # (1,4)+(2,5)=n11:3, (2,n9)+(2,n10)=n12:4, (3,6)+(3,n11)=n13:6,
+        # (4,n12)+(5,7)=n14:9, (6,n13)+(8,8)=n15:14, n14+n15=root.  Depths:
+        # n14, n15 1; n12, sym7, n13, sym8 2; n9, n10, sym6, n11 3; syms 0-5 4.
+        {"name": "halve_twice_limit4", "freq": [1, 1, 2, 3, 5, 8, 13, 21, 34], "limit": 4,
+         "expect": [4, 4, 4, 4, 4, 4, 3, 2, 2]},
+        # Limit 15 on 17 Fibonacci frequencies (natural depth 16), expected
+        # lengths from heap_code_lengths (tests/_vectors.py: a direct statement
+        # of the reference's BinaryHeap loop, independent of the oracle's C code).
+        {"name": "halve_limit15_fib17", "freq": FIB[:17], "limit": 15,
+         "expect": heap_code_lengths(FIB[:17], 15)},
     ]
+
+
+FIB = [1, 1]
+while len(FIB) < 24:
+    FIB.append(FIB[-1] + FIB[-2])
+
+
+# Token streams whose Huffman trees exceed the length limits, so that the
+# halving loop runs for the litlen code (limit 15), the distance code (limit
+# 15) and the code-length code (limit 7).  Stored as symbol frequencies:
+# `ll` counts per litlen symbol (0-255 literal bytes, 257-285 length codes;
+# 256 is the end of block, added by the encoder) and `dc` counts per
+# distance code; tests expand them with expand_tokens() in tests/_vectors.py.
+def encode_halving_cases():
+    cases = []
+    # litlen: bytes 0..18 with counts 1, 2, 3, 5, ...; with the end-of-block
+    # symbol (count 1) the litlen counts are Fibonacci numbers (natural depth 18)
+    ll = [0] * 286
+    for i in range(19):
+        ll[i] = FIB[i + 1]
+    cases.append({"name": "litlen_depth_gt15", "ll": ll, "dc": [0] * 30})
+    # distance: 20 distance codes with Fibonacci counts, all matches of length 3
+    # (length code 257), plus one literal
+    dc = [0] * 30
+    for i in range(20):
+        dc[i] = FIB[i]
+    ll = [0] * 286
+    ll[257] = sum(dc)
+    ll[65] = 1
+    cases.append({"name": "dist_depth_gt15", "ll": ll, "dc": dc})
+    # code-length code: a stream whose combined litlen/distance code lengths
+    # RLE into meta-symbol counts of natural depth 8 (> 7).  Found by a seeded
+    # search: litlen counts 2000 * r^-j (floored, at least 1) over a random
+    # order of the litlen symbols, distance counts from a Dirichlet draw.
+    ll, dc = meta_depth8_case()
+    cases.append({"name": "meta_depth_gt7", "ll": ll, "dc": dc})
+    return cases
+
+
+def meta_depth8_case():
+    """Trial 17 of the seeded search (numpy PCG64, seed 0); earlier trials are
+    replayed only to advance the generator.  The property (meta code of
+    natural depth > 7) is checked by tests/test_oracle.py on the encoded
+    header."""
+    import numpy as np
+
+    rng = np.random.default_rng(0)
+    for trial in range(18):
+        r = rng.uniform(1.02, 1.3)
+        nused = rng.integers(40, 286)
+        ranks = rng.permutation(286)
+        ll = np.zeros(286, np.int64)
+        for j, s in enumerate(ranks[:nused]):
+            if s != 256:
+                ll[s] = max(1, int(2000 * r ** (-j)))
+        nm = int(ll[257:286].sum())
+        dc = np.zeros(30, np.int64)
+        if nm:
+            w = rng.dirichlet(np.ones(30) * 0.3)
+            dc = np.floor(w * nm).astype(np.int64)
+            dc[0] += nm - dc.sum()
+    return [int(x) for x in ll], [int(x) for x in dc]
 
 
 def main():
@@ -220,6 +347,7 @@ def main():
             for c in lazy_cases()
         ],
         "huffman": huffman_cases(),
+        "encode_halving": encode_halving_cases(),
         # empty token stream: litlen {256:1} -> lens[0]=lens[256]=1 (fix-up),
         # dist none -> lens[0]=lens[1]=1; HLIT 0, HDIST 1, RLE 1,18(127),18(127),
         # 17(..)... ; verified with zlib inflate and gzip -t.

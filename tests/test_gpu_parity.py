@@ -10,6 +10,8 @@ import numpy as np
 import pytest
 
 import gpzip_amd
+from _vectors import expand_tokens
+from gpzip_amd import workloads
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -294,7 +296,7 @@ def test_server_alone_equals_mix(ctx):
             for i, s in enumerate([1000, 300000, 5 << 20, 4096, 2 << 20, 123457])]
     say("server alone")
     alone = {}
-    srv = gpzip_amd.Server(ctx, streams=2)
+    srv = gpzip_amd.Server(ctx, streams=2, budget_bytes=2 << 30)
     try:
         for i, r in enumerate(reqs):
             srv.submit(r, i)
@@ -353,3 +355,248 @@ def test_determinism_repeat(ctx):
     a = ctx.compress_chunk(data)
     for _ in range(3):
         assert ctx.compress_chunk(data) == a
+
+
+@pytest.mark.parametrize("case", GOLD["encode_halving"], ids=lambda c: c["name"])
+def test_encode_block_halving_cases(ctx, case):
+    """Streams whose litlen, distance and code-length codes exceed their
+    limits: k_huffman's halving loop runs for each, bytes equal the oracle."""
+    toks = expand_tokens(case["ll"], case["dc"])
+    say("encode_block halving", case["name"], toks.size)
+    got = ctx.encode_block(toks)
+    exp = O.encode_block(toks)
+    assert got == exp, f"first byte difference at {first_diff(np.frombuffer(got, np.uint8), np.frombuffer(exp, np.uint8))}"
+
+
+def test_encode_block_match_heavy_stream(ctx):
+    """300,000 long far matches (lengths 131-257, distances 16385-32768: up to
+    48 bits each) encode to more than 2 bytes per token; the output region is
+    sized for 48 bits per token, and the bytes equal the oracle."""
+    rng = np.random.default_rng(31)
+    n = 300_000
+    toks = ((rng.integers(131, 258, n).astype(np.uint32) << 16) | rng.integers(16385, 32769, n).astype(np.uint32))
+    say("encode_block match-heavy", n)
+    got = ctx.encode_block(toks)
+    exp = O.encode_block(toks)
+    assert len(exp) > 2 * n + 1024
+    assert got == exp, f"first byte difference at {first_diff(np.frombuffer(got, np.uint8), np.frombuffer(exp, np.uint8))}"
+
+
+@pytest.mark.parametrize("kind,n", [("text", 1 << 20), ("random", (1 << 22) - 2), ("log", (1 << 22) + 4097)])
+def test_large_member_equals_oracle(ctx, kind, n):
+    """Members above the 128 KiB split: 256+ segments (k_chain's uncached
+    path), and on both sides of 2^22 - 1 tokens, where k_huffman's heap keys
+    go from 32 to 64 bits."""
+    data = (np.random.default_rng(n).integers(0, 256, n, dtype=np.uint8) if kind == "random"
+            else gpzip_amd.synth(kind, n, 5 + n))
+    say("large member", kind, n)
+    got = ctx.gzip_member(data)
+    exp = O.gzip_member(data)
+    assert got == exp, f"first byte difference at {first_diff(np.frombuffer(got, np.uint8), np.frombuffer(exp, np.uint8))}"
+
+
+def test_member_at_the_16mib_cap(ctx):
+    """A 16 MiB - 1 member (the largest any entry point accepts; the oracle
+    would take minutes): it decodes to the input, its block equals
+    gpz_encode_block of gpz_walk of gpz_match_find (the stage entry points),
+    and its litlen code lengths are the limited heap-Huffman lengths of the
+    walked tokens' histogram."""
+    from _vectors import heap_code_lengths
+    from test_oracle import _parse_dynamic_header
+
+    n = (16 << 20) - 1
+    data = gpzip_amd.synth("text", n, 16)
+    say("member at the cap", n)
+    gz = ctx.gzip_member(data)
+    assert gzip.decompress(gz) == data.tobytes()
+    walked = ctx.walk(ctx.match_find(data), data)
+    blk = ctx.encode_block(walked)
+    assert gz[10:-8] == blk
+    lens = walked >> 16
+    lit = walked[lens == 0] & 0xFF
+    ll = np.bincount(lit, minlength=286)
+    ml = lens[lens != 0].astype(np.int64)
+    base = [3, 4, 5, 6, 7, 8, 9, 10, 11, 13, 15, 17, 19, 23, 27, 31, 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195,
+            227, 258]
+    code = np.searchsorted(np.array(base), ml, side="right") - 1
+    ll[257:286] += np.bincount(code, minlength=29)
+    ll[256] += 1
+    hlit, hdist, mlens, syms, hl = _parse_dynamic_header(blk)
+    exp = heap_code_lengths([int(x) for x in ll], 15)
+    assert hl[:hlit] == exp[:hlit] and not any(exp[hlit:])
+
+
+def test_compress_chunk_with_1mib_members():
+    """A context whose members are 1 MiB (256 segments each): plain and
+    batched forms equal the oracle's 1 MiB split."""
+    c = gpzip_amd.Context(sub_chunk=1 << 20)
+    try:
+        chunks = [gpzip_amd.synth(k, s, 60 + i) for i, (k, s) in
+                  enumerate([("text", (3 << 20) + 777), ("binmix", 1 << 20), ("log", (2 << 20) - 5)])]
+        say("1 MiB members")
+        ref = [O.gzip_chunk(x, 1 << 20) for x in chunks]
+        for x, r in zip(chunks, ref):
+            assert c.compress_chunk(x) == r
+        assert c.batch(chunks) == ref
+    finally:
+        c.close()
+
+
+def _oracle_many(chunks, sub=131072):
+    import concurrent.futures as cf
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        return list(ex.map(lambda x: O.gzip_chunk(x, sub), chunks))
+
+
+def test_mixed_sizes_sample_equals_oracle(ctx):
+    """96 chunks of the mixed-size workload (4 KiB-4 MiB, text/binmix/log),
+    the 8 largest among them: the autotuned batch == the plain form == the
+    oracle."""
+    import torch
+
+    data, sizes = workloads.mixed_sizes_workload()
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    order = np.argsort(sizes)
+    pick = sorted(set(order[-8:].tolist()) | set(range(0, len(sizes), 46)))[:96]
+    chunks = [np.ascontiguousarray(data[offs[i]:offs[i] + sizes[i]]) for i in pick]
+    say("mixed sample", len(chunks), sum(c.size for c in chunks))
+    ref = _oracle_many(chunks)
+    # autotune on the sample, device-resident
+    lay, o = [], 0
+    for c in chunks:
+        lay.append(o)
+        o += (c.size + 255) // 256 * 256
+    host = np.zeros(o + 256, np.uint8)
+    for a, c in zip(lay, chunks):
+        host[a:a + c.size] = c
+    d_in = torch.from_numpy(host).to("cuda:0")
+    d_out = torch.zeros(3 * o + (1 << 24), dtype=torch.uint8, device="cuda:0")
+    tune, _, _ = ctx.autotune(list(zip(lay, [c.size for c in chunks])), d_in.data_ptr(), d_out.data_ptr())
+    del d_in, d_out
+    ctx.set_tune(tune)
+    try:
+        assert ctx.batch(chunks) == ref
+    finally:
+        ctx.set_tune(None)
+    for c, r in zip(chunks, ref):
+        assert ctx.compress_chunk(c) == r
+
+
+def test_mixed_sizes_full_batch_equals_plain(ctx):
+    """All 4,096 chunks (2.5 GB) as one batch equal the plain form chunk by chunk."""
+    import concurrent.futures as cf
+
+    data, sizes = workloads.mixed_sizes_workload()
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    chunks = [data[offs[i]:offs[i] + sizes[i]] for i in range(len(sizes))]
+    say("mixed full batch", len(chunks))
+    got = ctx.batch(chunks)
+    say("mixed full plain")
+    with cf.ThreadPoolExecutor(8) as ex:
+        plain = list(ex.map(ctx.compress_chunk, chunks))
+    bad = [i for i, (a, b) in enumerate(zip(got, plain)) if a != b]
+    assert not bad, f"{len(bad)} chunks differ, first {bad[:5]}"
+
+
+def test_serving_requests_equal_plain(ctx):
+    """500 requests of the serving workload (log-normal sizes, Poisson
+    arrivals at 4 GB/s) from 2 submitting threads into a 4-lane server: every
+    result equals the plain form of that request, 24 of them equal the
+    oracle."""
+    import threading
+    import time
+
+    n = 500
+    R = workloads.ServingRequests(n, 4.0)
+    srv = gpzip_amd.Server(ctx, streams=4, budget_bytes=4 << 30)
+    results = {}
+    errors = []
+
+    def submit(t):
+        try:
+            for i in range(t, n, 2):
+                now = time.perf_counter()
+                if t0 + R.arrive[i] > now:
+                    time.sleep(t0 + R.arrive[i] - now)
+                srv.submit(R.data(i), i)
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    say("serving", n, int(R.sizes.sum()))
+    t0 = time.perf_counter()
+    subs = [threading.Thread(target=submit, args=(t,)) for t in range(2)]
+    for s_ in subs:
+        s_.start()
+    try:
+        while len(results) < n:
+            r = srv.next(timeout_ms=10000)
+            assert r is not None, "no result within 10 s"
+            results[r[0]] = r[1]
+    finally:
+        for s_ in subs:
+            s_.join()
+        srv.stop()
+    assert not errors, errors
+    for i in range(n):
+        assert results[i] == ctx.compress_chunk(R.data(i)), f"request {i} ({R.sizes[i]} bytes)"
+    sample = list(np.argsort(R.sizes)[:16]) + list(range(0, n, 63))
+    ref = _oracle_many([R.data(int(i)) for i in sample])
+    for i, r in zip(sample, ref):
+        assert results[int(i)] == r, f"request {i} differs from the oracle"
+
+
+def test_server_admission_blocks_at_budget(ctx):
+    """With an input pool of 12 MiB, 40 requests of 1-4 MiB submitted from two
+    threads wait for pool space (admission blocks, first come first served);
+    peak arena use stays within the budget; results equal the plain form; a
+    request larger than the pool is refused; a tag in use is refused."""
+    import threading
+
+    probe = gpzip_amd.Server(ctx, streams=2, max_batch_bytes=4 << 20, budget_bytes=1 << 30)
+    ws = probe.stats()["workspace_bytes"]
+    probe.stop()
+    budget = ws + (12 << 20)
+    srv = gpzip_amd.Server(ctx, streams=2, max_batch_bytes=4 << 20, budget_bytes=budget)
+    rng = np.random.default_rng(8)
+    kinds = ["text", "binmix", "log"]
+    reqs = [gpzip_amd.synth(kinds[i % 3], int(rng.integers(1 << 20, 4 << 20)), 300 + i) for i in range(40)]
+    say("admission", budget, sum(r.size for r in reqs))
+    with pytest.raises(gpzip_amd.GpzError) as e:
+        srv.submit(np.zeros(16 << 20, np.uint8), 999)
+    assert e.value.status == gpzip_amd.GPZ_E_BUSY
+    results = {}
+    errors = []
+
+    def submit(t):
+        try:
+            for i in range(t, len(reqs), 2):
+                srv.submit(reqs[i], i)
+        except Exception as ex:  # noqa: BLE001
+            errors.append(ex)
+
+    subs = [threading.Thread(target=submit, args=(t,)) for t in range(2)]
+    for s_ in subs:
+        s_.start()
+    try:
+        while len(results) < len(reqs):
+            r = srv.next(timeout_ms=20000)
+            assert r is not None, "no result within 20 s"
+            results[r[0]] = r[1]
+        for s_ in subs:
+            s_.join()
+        # a tag that is live (submitted, not yet released) is refused
+        srv.submit(reqs[0][:1000], 7000)
+        with pytest.raises(gpzip_amd.GpzError):
+            srv.submit(reqs[1][:1000], 7000)
+        assert srv.next(timeout_ms=20000)[0] == 7000
+        st = srv.stats()
+    finally:
+        for s_ in subs:
+            s_.join()
+        srv.stop()
+    assert not errors, errors
+    assert st["arena_bytes"] >= budget and st["peak_bytes"] <= st["arena_bytes"]
+    assert st["waited"] > 0, "no submission waited for pool space"
+    for i, r in enumerate(reqs):
+        assert results[i] == ctx.compress_chunk(r), f"request {i}"

@@ -267,3 +267,121 @@ def test_gzip_empty_member():
     assert gzip.decompress(gz) == b""
     assert gz[:10] == bytes.fromhex("1f8b08000000000000ff")
     assert gz[10:22].hex() == GOLD["encode_empty_hex"]
+
+
+# ---------------------------------------------------------------------------
+# Huffman halving and hash3, pinned so that plausible mistakes fail
+
+from _vectors import expand_tokens, heap_code_lengths, heap_depths, inflate_with_zero_window, \
+    reconstruct_with_zero_window  # noqa: E402
+
+
+def test_huffman_matches_heap_model_random():
+    """The oracle's code lengths equal a heap statement of huffman.rs on
+    random and skewed vectors, many of which need the halving loop."""
+    rng = np.random.default_rng(12)
+    halved = 0
+    for i in range(1500):
+        n = int(rng.integers(2, 287))
+        kind = i % 3
+        if kind == 0:
+            f = rng.integers(0, 1000, n)
+        elif kind == 1:  # geometric: deep trees
+            f = np.floor(1e6 * rng.uniform(1.2, 2.5) ** -rng.permutation(n).astype(float)).astype(np.int64) + 1
+        else:  # sparse with zeros
+            f = rng.integers(0, 50, n) * (rng.random(n) < 0.3)
+        limit = 7 if (i % 5 == 0 and n <= 19) else 15
+        if limit == 7:
+            f = f[:19]
+        f = [int(x) for x in f]
+        exp = heap_code_lengths(f, limit)
+        if sum(1 for x in f if x) > 1 and max(heap_depths(f)) > limit:
+            halved += 1
+        assert list(O.build_code_lengths(f, limit)) == exp, (i, f, limit)
+    assert halved > 100
+
+
+def _meta_freq(syms):
+    mf = [0] * 19
+    for s, _ in syms:
+        mf[s] += 1
+    return mf
+
+
+@pytest.mark.parametrize("case", GOLD["encode_halving"], ids=lambda c: c["name"])
+def test_encode_halving_cases(case):
+    """Each stream needs the halving loop in the code it targets (natural
+    depth above the limit); the oracle's header carries the heap model's
+    limited lengths; the block decodes to the tokens."""
+    toks = expand_tokens(case["ll"], case["dc"])
+    blk = O.encode_block(toks)
+    hlit, hdist, mlens, syms, lens = _parse_dynamic_header(blk)
+    ll = list(case["ll"])
+    ll[256] += 1  # end of block
+    ll_lens = heap_code_lengths(ll, 15)
+    if sum(1 for x in ll if x) < 2:
+        ll_lens[0] = ll_lens[256] = 1
+    d_lens = heap_code_lengths(case["dc"], 15)
+    used_d = [i for i, x in enumerate(case["dc"]) if x]
+    if not used_d:
+        d_lens[0] = d_lens[1] = 1
+    elif len(used_d) == 1:
+        d_lens[used_d[0]] = 1
+        d_lens[1 - min(used_d[0], 1)] = 1
+    assert lens[:hlit] == ll_lens[:hlit] and not any(ll_lens[hlit:])
+    assert lens[hlit:] == d_lens[:hdist] and not any(d_lens[hdist:])
+    mf = _meta_freq(syms)
+    assert [mlens[s] for s in range(19)] == heap_code_lengths(mf, 7)
+    natural = {"litlen_depth_gt15": (max(heap_depths(ll)), 15),
+               "dist_depth_gt15": (max(heap_depths(case["dc"])), 15),
+               "meta_depth_gt7": (max(heap_depths(mf)), 7)}[case["name"]]
+    assert natural[0] > natural[1], f"{case['name']}: natural depth {natural[0]} does not exceed {natural[1]}"
+    assert inflate_with_zero_window(blk) == reconstruct_with_zero_window(toks)
+
+
+def _build_mutant(tmp_path, name, defines):
+    src = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "gpzip_oracle.c")
+    so = str(tmp_path / f"liboracle_{name}.so")
+    subprocess.run(["gcc", "-O1", "-fPIC", "-shared", "-std=c11", *defines, "-o", so, src], check=True)
+    import ctypes
+
+    L = ctypes.CDLL(so)
+    L.orc_match_find.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p]
+    L.orc_build_code_lengths.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p]
+    return L
+
+
+def _kat_ok(L, kat) -> bool:
+    data = np.frombuffer(base64.b64decode(kat["input_b64"]), np.uint8)
+    t = np.zeros(data.size, np.uint32)
+    L.orc_match_find(data.ctypes.data, data.size, 32768, t.ctypes.data)
+    return all((int(t[int(p)]) >> 16, int(t[int(p)]) & 0xFFFF) == tuple(v) for p, v in kat["expect"].items())
+
+
+@pytest.mark.parametrize("name,defines", [("mul", ["-DORC_HASH_MUL=0x9E3779B9u"]),
+                                          ("reversed", ["-DORC_HASH_REVERSED"])])
+def test_hash3_mutants_fail_collision_kats(tmp_path, name, defines):
+    """A build of the oracle with another multiplier or byte order passes
+    the collision-free known answers but fails the collision ones."""
+    kats = {k["name"]: k for k in GOLD["match_kats"]}
+    L = _build_mutant(tmp_path, name, defines)
+    assert _kat_ok(L, kats["newest_wins_tie"])  # collision-free cases cannot tell
+    assert not _kat_ok(L, kats["hash3_collisions_shadow"])
+    good = _build_mutant(tmp_path, "good", [])
+    assert _kat_ok(good, kats["hash3_collisions_shadow"])
+
+
+def test_halving_mutant_fails_halving_kats(tmp_path):
+    """A build that rounds the halving step up passes the cases that need no
+    halving but fails the hand-derived halving ones."""
+    L = _build_mutant(tmp_path, "halve_up", ["-DORC_HALVE(f)=(((f)+1u)/2u)"])
+    cases = {c["name"]: c for c in GOLD["huffman"]}
+
+    def lens(c):
+        f = np.array(c["freq"], np.uint32)
+        out = np.zeros(f.size, np.uint8)
+        L.orc_build_code_lengths(f.ctypes.data, f.size, c["limit"], out.ctypes.data)
+        return list(out)
+
+    assert lens(cases["tie_leaf_first"]) == cases["tie_leaf_first"]["expect"]
+    assert lens(cases["halve_once_limit7"]) != cases["halve_once_limit7"]["expect"]
