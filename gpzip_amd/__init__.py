@@ -23,6 +23,7 @@ GPZ_PLAN_GRAPH, GPZ_PLAN_PERSISTENT, GPZ_PLAN_TIMED = 1, 2, 4
 ABI_SYMBOLS = [
     "gpz_config_default", "gpz_init", "gpz_destroy", "gpz_last_error", "gpz_status_string",
     "gpz_device_count", "gpz_device_label", "gpz_compress_bound", "gpz_compress_chunk",
+    "gpz_compress_chunk_noalloc",
     "gpz_match_find", "gpz_walk", "gpz_encode_block", "gpz_gzip_member",
     "gpz_batch_submit", "gpz_batch_wait", "gpz_batch_result", "gpz_batch_free",
     "gpz_plan_create", "gpz_plan_out_capacity", "gpz_plan_run", "gpz_plan_sync", "gpz_plan_result",
@@ -88,6 +89,7 @@ def load_library() -> C.CDLL:
         "gpz_device_label": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
         "gpz_compress_bound": (C.c_size_t, [vp, C.c_size_t]),
         "gpz_compress_chunk": (C.c_int, [vp, vp, C.c_size_t, vp, C.c_size_t, szp]),
+        "gpz_compress_chunk_noalloc": (C.c_int, [vp, vp, C.c_size_t, vp, C.c_size_t, szp]),
         "gpz_match_find": (C.c_int, [vp, vp, C.c_size_t, vp]),
         "gpz_walk": (C.c_int, [vp, vp, vp, C.c_size_t, C.c_int, vp, szp]),
         "gpz_encode_block": (C.c_int, [vp, vp, C.c_size_t, vp, C.c_size_t, szp]),

@@ -1780,6 +1780,40 @@ __global__ __launch_bounds__(BT) void k_emit(Work w) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_plan_one: the member and segment tables of one chunk of len bytes cut into
+// members of `sub` bytes, written on the device (the host's build_plan, in
+// closed form: every member but the last is full).  For callers whose
+// descriptor tables live in memory the host does not stage.
+
+__global__ __launch_bounds__(256) void k_plan_one(Work w, uint32_t len, uint32_t sub) {
+    MemberDesc *mem = const_cast<MemberDesc *>(w.mem);
+    uint32_t *seg_mem = const_cast<uint32_t *>(w.seg_mem);
+    uint32_t *chunk_mem0 = const_cast<uint32_t *>(w.chunk_mem0);
+    Counts *counts = const_cast<Counts *>(w.counts);
+    const uint32_t M = len <= sub ? 1u : (len + sub - 1) / sub;
+    const uint32_t spm = (sub + SEG - 1) >> SEG_LOG2;  // segments of a full member
+    const uint32_t pstep = (sub + POS_ALIGN - 1) / POS_ALIGN * POS_ALIGN;
+    const uint32_t nlast = len <= sub ? len : len - (M - 1) * sub;
+    const uint32_t G = (M - 1) * spm + ((nlast + SEG - 1) >> SEG_LOG2);
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+        MemberDesc d;
+        d.n = i + 1 == M ? nlast : sub;
+        d.in_off = (uint64_t)i * sub;
+        d.pos_base = (uint64_t)i * pstep;
+        d.seg_base = i * spm;
+        d.nseg = (d.n + SEG - 1) >> SEG_LOG2;
+        d.chunk = 0;
+        mem[i] = d;
+    }
+    for (uint32_t g = threadIdx.x; g < G; g += blockDim.x) seg_mem[g] = g / spm;
+    if (threadIdx.x == 0) {
+        chunk_mem0[0] = 0;
+        chunk_mem0[1] = M;
+        *counts = Counts{G, M, 1, 0};
+    }
+}
+
 }  // namespace gpz
 
 // ---------------------------------------------------------------------------
@@ -1850,6 +1884,9 @@ void launch_huffman(const Work &w, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_huffman, dim3(grid), dim3(64), 0, st, w);
 }
 void launch_scan(const Work &w, hipStream_t st) { hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, w); }
+void launch_plan_one(const Work &w, uint32_t len, uint32_t sub, hipStream_t st) {
+    hipLaunchKernelGGL(k_plan_one, dim3(1), dim3(256), 0, st, w, len, sub);
+}
 void launch_emit(const Work &w, uint32_t grid, int variant, hipStream_t st) {
     if (variant == 1)
         hipLaunchKernelGGL(k_emit<256>, dim3(grid), dim3(256), 0, st, w);

@@ -83,6 +83,7 @@ struct HostBuf {
 };
 
 Layout layout_for(const Caps &c);
+Work work_at(const Layout &L, uint8_t *b);
 
 struct Workspace {
     DevBuf ws;

@@ -37,6 +37,7 @@ void launch_hist_tokens(const Work &w, uint32_t grid, hipStream_t st);
 void launch_huffman(const Work &w, uint32_t grid, hipStream_t st);
 void launch_scan(const Work &w, hipStream_t st);
 void launch_emit(const Work &w, uint32_t grid, int variant, hipStream_t st);
+void launch_plan_one(const Work &w, uint32_t len, uint32_t sub, hipStream_t st);
 
 static thread_local char g_err[512];
 
@@ -187,11 +188,9 @@ Layout layout_for(const Caps &c) {
     return L;
 }
 
-// Point the kernel arguments at a workspace laid out by `lay` from b.
-void Workspace::point(uint8_t *b) {
-    const Layout &L = lay;
-    Work &w = work;
-    w = Work{};
+// Kernel arguments for a workspace laid out by L from device address b.
+Work work_at(const Layout &L, uint8_t *b) {
+    Work w{};
     w.mem = (const MemberDesc *)(b + L.mem);
     w.seg_mem = (const uint32_t *)(b + L.seg_mem);
     w.chunk_mem0 = (const uint32_t *)(b + L.chunk_mem0);
@@ -217,6 +216,11 @@ void Workspace::point(uint8_t *b) {
     w.total = (uint64_t *)(b + L.total);
     w.out = b + L.out;
     w.window = 32768;
+    return w;
+}
+
+void Workspace::point(uint8_t *b) {
+    work = work_at(lay, b);
     base = b;
 }
 
@@ -356,6 +360,37 @@ int Lane::init(int dev) {
     return GPZ_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (gpz_compress_chunk_noalloc):
+// the workspace of one chunk of at most NOALLOC_MAX_BYTES plus its input and
+// output staging, in a file-scope device array.  Every GPU the library runs
+// on holds its own copy until the process ends.  In host code `scratch`
+// names a host-side stand-in, not a device address: the device address of
+// each GPU's copy comes from hipGetSymbolAddress with that GPU current (done
+// once per device at gpz_init) and is the only one ever used.
+constexpr uint64_t NOALLOC_MAX_BYTES = 2u << 20;
+constexpr uint64_t NOALLOC_SCRATCH_BYTES = 40u << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[MAX_DEVICES];
+
+// Device address of `scratch` on device dev (current).  GPZ_OK or GPZ_E_DEVICE.
+static int scratch_address(int dev, void **out) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(GPZ_E_NODEV, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GPZ_E_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GPZ_OK;
+}
+
 }  // namespace gpz
 
 using namespace gpz;
@@ -370,6 +405,13 @@ struct gpz_ctx {
     std::condition_variable cv;
     std::vector<std::unique_ptr<Lane>> lanes;
     std::vector<Lane *> free_lanes;
+    void *scratch = nullptr;               // this device's copy of the no-alloc scratch
+    hipStream_t noalloc_st = nullptr;      // stream of gpz_compress_chunk_noalloc
+    std::mutex noalloc_mu;                 // one no-alloc call at a time per context
+
+    ~gpz_ctx() {
+        if (noalloc_st) (void)hipStreamDestroy(noalloc_st);
+    }
 
     Lane *acquire() {
         std::unique_lock<std::mutex> lk(mu);
@@ -546,6 +588,12 @@ int gpz_init(const gpz_config *cfg_in, gpz_ctx **out) {
     ctx->device = dev;
     ctx->n_cu = prop.multiProcessorCount;
     ctx->cfg = cfg;
+    {
+        int rc = scratch_address(dev, &ctx->scratch);
+        if (rc) return rc;
+        if (hipStreamCreateWithFlags(&ctx->noalloc_st, hipStreamNonBlocking) != hipSuccess)
+            return fail(GPZ_E_DEVICE, "stream creation failed");
+    }
     snprintf(ctx->label, sizeof(ctx->label), "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
     for (uint32_t i = 0; i < cfg.lanes; i++) {
         auto ln = std::make_unique<Lane>();
@@ -592,6 +640,57 @@ int gpz_compress_chunk(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out
     if (rc) hipStreamSynchronize(ln->st);
     ctx->release(ln);
     return rc;
+}
+
+int gpz_compress_chunk_noalloc(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out, size_t out_cap,
+                               size_t *out_len) {
+    clear_error();
+    if (!ctx || (!in && len) || !out_len) return fail(GPZ_E_INVAL, "bad argument");
+    if (len > NOALLOC_MAX_BYTES)
+        return fail(GPZ_E_INVAL, "chunk of %zu bytes exceeds the no-alloc limit of %llu bytes", len,
+                    (unsigned long long)NOALLOC_MAX_BYTES);
+    // the plan in closed form (k_plan_one writes the same tables on the device)
+    const uint64_t sub = ctx->cfg.sub_chunk;
+    const uint64_t M = len <= sub ? 1 : (len + sub - 1) / sub;
+    const uint64_t nlast = len <= sub ? len : len - (M - 1) * sub;
+    const uint64_t spm = (sub + SEG - 1) >> SEG_LOG2, pstep = round_up(sub, POS_ALIGN);
+    Caps c;
+    c.M = M;
+    c.G = (M - 1) * spm + ((nlast + SEG - 1) >> SEG_LOG2);
+    c.P = (M - 1) * pstep + round_up(nlast, POS_ALIGN) + POS_ALIGN;
+    c.C = 1;
+    c.out = round_up(2 * (uint64_t)len + M * (1024 + 18), 256) + 256;
+    const Layout L = layout_for(c);
+    const uint64_t in_at = round_up(L.end, 256), in_bytes = round_up(len, 256) + 256;
+    if (in_at + in_bytes > NOALLOC_SCRATCH_BYTES)
+        return fail(GPZ_E_INVAL, "chunk of %zu bytes in %llu-byte members needs %llu bytes of scratch (%llu)", len,
+                    (unsigned long long)sub, (unsigned long long)(in_at + in_bytes),
+                    (unsigned long long)NOALLOC_SCRATCH_BYTES);
+    std::lock_guard<std::mutex> lk(ctx->noalloc_mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->noalloc_st;
+    uint8_t *base = (uint8_t *)ctx->scratch;
+    Work w = work_at(L, base);
+    w.in = base + in_at;
+    w.gzip = 1;
+    w.strict = 0;
+    // no array needs to start at zero: every kernel writes what it later
+    // reads (k_chain clears the histograms), as on reused lane workspaces
+    if (len) HIPCHK(hipMemcpyAsync(base + in_at, in, len, hipMemcpyDefault, st));
+    launch_plan_one(w, (uint32_t)len, (uint32_t)sub, st);
+    enqueue(w, grids_for(c, ctx->n_cu), MODE_FULL, st, nullptr);
+    HIPCHK(hipGetLastError());
+    uint64_t meta[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&meta[0], w.chunk_off, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&meta[1], w.chunk_len, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *out_len = meta[1];
+    if (meta[1] > out_cap) return fail(GPZ_E_NOSPACE, "output needs %llu bytes", (unsigned long long)meta[1]);
+    if (meta[1]) {
+        HIPCHK(hipMemcpyAsync(out, w.out + meta[0], meta[1], hipMemcpyDefault, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return GPZ_OK;
 }
 
 int gpz_gzip_member(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out, size_t out_cap,

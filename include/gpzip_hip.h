@@ -101,6 +101,19 @@ size_t gpz_compress_bound(const gpz_ctx *ctx, size_t len);
 int gpz_compress_chunk(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out, size_t out_cap,
                        size_t *out_len);
 
+/* The same compression with no allocation inside the call: no hipMalloc,
+ * hipHostMalloc or free, for callers that cannot allocate (tight serving
+ * loops, harnesses that forbid it).  No counterpart in the reference.  The
+ * workspace, input staging and output staging come from a fixed 40 MiB
+ * device array that each GPU holds until the process ends, which limits a
+ * call to len <= 2 MiB (GPZ_E_INVAL above that, or when the context's member
+ * size needs more scratch).  in and out may be host or device memory.  Calls
+ * through one context are serialized; calls on one GPU from different
+ * contexts share the array and race.  Output bytes equal
+ * gpz_compress_chunk's. */
+int gpz_compress_chunk_noalloc(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out, size_t out_cap,
+                               size_t *out_len);
+
 /* ---- stage-level entry points (parity / debugging) --------------------- */
 
 /* Per-position packed tokens for one member (len <= 16 MiB). */
