@@ -414,6 +414,42 @@ def serve(ctx, n_requests, load_gbps, seed=4, submitters=4, budget_bytes=64 * 10
             "peak_arena_bytes": stats["peak_bytes"], "admissions_waited": stats["waited"]}
 
 
+def hybrid_sweep(mib: int = 256, settings=((0, 0, 4), (1, 0, 4), (8, 0, 4), (16, 0, 4), (8, 1000, 4), (4, 1000, 0),
+                                           (8, 1000, 0), (16, 1000, 0), (8, 0, 0))) -> dict:
+    """`--backend auto` end to end: the C++ HybridBackend (cli/src/hybrid.rs
+    mirror) compresses the binmix corpus from host memory through a
+    ParallelChunkedWriter of 2 MiB chunks, CPU workers = host cores, with 0..16
+    GPU permits; every stream is decoded and checked."""
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(ROOT, "tests", "cpp", "test_backend")
+    src = exe + ".cpp"
+    lib = gpzip_amd.HIP_LIB_PATH
+    if not os.path.exists(exe) or os.path.getmtime(exe) < max(os.path.getmtime(src), os.path.getmtime(lib)):
+        subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src, "-o", exe, lib,
+                        "-Wl,-rpath," + os.path.dirname(lib), "-lpthread"], check=True)
+    data = gpzip_amd.synth("binmix", mib * MiB, 1)
+    res = []
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "in.bin")
+        data.tofile(f)
+        for p, wait, warmup in settings:
+            log(f"hybrid: {p} GPU permits, wait {wait} ms, warm-up {warmup} ...")
+            r = subprocess.run([exe, "hybrid", f, str(p), str(warmup), str(wait)], capture_output=True, timeout=600)
+            if r.returncode:
+                raise RuntimeError(r.stderr.decode())
+            kv = r.stderr.decode().split()
+            st = dict(zip(kv[0::2], kv[1::2]))
+            assert gpzip_amd.gunzip(r.stdout) == data.tobytes(), "hybrid stream does not decode"
+            g, c = int(st["gpu_chunks"]), int(st["cpu_chunks"])
+            res.append({"gpu_permits": p, "gpu_wait_ms": wait, "warmup_chunks": warmup, "gbps": round(data.size / float(st["seconds"]) / 1e9, 3),
+                        "gpu_chunk_share": round(g / max(1, g + c), 3), "ratio": round(len(r.stdout) / data.size, 5)})
+    cores, _ = host_cores()
+    return {"workload": f"`--backend auto`: binmix {mib} MiB from host memory, 2 MiB chunks, {cores} CPU workers "
+                        f"(zlib level 5) + GPU permits", "runs": res}
+
+
 def mixed_sizes(torch, ctx, steps):
     """Config 3: the batched form against the plain form on 4,096 chunks of
     4 KiB-4 MiB (outputs equal chunk by chunk, the whole output decodes, the
@@ -447,6 +483,9 @@ def main():
     ap.add_argument("--serve-load", type=float, nargs="+", default=[8.0],
                     help="offered loads (GB/s) of the serving runs; the first goes into the headline line, "
                          "the others are extra JSON lines")
+    ap.add_argument("--hybrid", action="store_true",
+                    help="only time `--backend auto` (the C++ HybridBackend, host buffers) on the binmix corpus "
+                         "for several GPU permit counts, one JSON line")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="leave out the host-buffer end-to-end rates")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -469,6 +508,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "elementwise uint8 copy y = x + 1", "read_bytes": nb,
                           "write_bytes": nb, "launches": 3}), flush=True)
+        return
+
+    if args.hybrid:
+        print(json.dumps(hybrid_sweep()), flush=True)
         return
 
     log("generating binmix 256 MiB ...")

@@ -30,6 +30,7 @@ ABI_SYMBOLS = [
     "gpz_plan_timing", "gpz_plan_destroy", "gpz_set_tune", "gpz_get_tune", "gpz_autotune",
     "gpz_server_config_default", "gpz_server_start", "gpz_request_submit", "gpz_request_next",
     "gpz_request_release", "gpz_server_stats", "gpz_server_stop",
+    "gpz_gzip_member_starts", "gpz_gunzip",
 ]
 
 
@@ -115,6 +116,8 @@ def load_library() -> C.CDLL:
         "gpz_request_release": (C.c_int, [vp, C.c_uint64]),
         "gpz_server_stats": (C.c_int, [vp, u64p, C.c_size_t]),
         "gpz_server_stop": (None, [vp]),
+        "gpz_gzip_member_starts": (C.c_int, [vp, C.c_size_t, vp, C.c_size_t, szp]),
+        "gpz_gunzip": (C.c_int, [vp, C.c_size_t, vp, C.c_size_t, szp, C.c_uint]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -325,6 +328,35 @@ class Server:
         if self.h:
             self.L.gpz_server_stop(self.h)
             self.h = None
+
+
+def member_starts(data) -> list[int]:
+    """Offsets of the fixed 10-byte gzip member headers (gpz_gzip_member_starts)."""
+    L = load_library()
+    a = _u8(data)
+    n = C.c_size_t()
+    rc = L.gpz_gzip_member_starts(_ptr(a), a.size, None, 0, C.byref(n))
+    if rc not in (GPZ_OK, GPZ_E_NOSPACE):
+        _check(rc)
+    out = np.zeros(max(n.value, 1), np.uint64)
+    _check(L.gpz_gzip_member_starts(_ptr(a), a.size, _ptr(out), out.size, C.byref(n)))
+    return [int(x) for x in out[: n.value]]
+
+
+def gunzip(data, threads: int = 0) -> bytes:
+    """Decode a gzip stream on the host cores (gpz_gunzip): members in parallel
+    when the header scan finds two or more, else a serial decode."""
+    L = load_library()
+    a = _u8(data)
+    n = C.c_size_t()
+    rc = L.gpz_gunzip(_ptr(a), a.size, None, 0, C.byref(n), threads)
+    if rc == GPZ_OK:
+        return b""
+    if rc != GPZ_E_NOSPACE:
+        _check(rc)
+    out = np.empty(n.value, np.uint8)
+    _check(L.gpz_gunzip(_ptr(a), a.size, _ptr(out), out.size, C.byref(n), threads))
+    return out[: n.value].tobytes()
 
 
 def synth(kind: str, n: int, seed: int) -> np.ndarray:

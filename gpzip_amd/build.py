@@ -20,6 +20,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["gpz_kernels.hip", "gpz_runtime.hip", "gpz_batch.hip", "gpz_server.hip", "gpz_backend.hip"]
+# host-only C++ (the CPU codec and decode side, the hybrid scheduler), built with g++
+CPP_SOURCES = ["gpz_cpu.cpp", "gpz_hybrid.cpp"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -57,10 +60,14 @@ def build_hip(verbose: bool = False) -> str:
         obj = os.path.join(BUILD, src + ".o")
         objs.append(obj)
         cmds.append([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
+    for src in CPP_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        objs.append(obj)
+        cmds.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         list(ex.map(_run, cmds))
     tmp = HIP_LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread", "-lz"])
     os.replace(tmp, HIP_LIB)
     if verbose:
         print("built", HIP_LIB)

@@ -31,22 +31,58 @@ void ParallelChunkedWriter::check_sticky() {
     if (sticky_) std::rethrow_exception(sticky_);
 }
 
-// Write every result that is next in index order.  Called with mu_ held.
+// Write every result that is next in index order.  Called by the producer
+// (the thread that calls write_all / finalize) with mu_ held; the writes
+// themselves run with mu_ released, so workers can post results meanwhile.
 void ParallelChunkedWriter::write_ready_locked(std::unique_lock<std::mutex> &lk) {
     for (;;) {
         auto it = pending_.find(write_index_);
-        if (it == pending_.end()) return;
+        if (it == pending_.end() || sticky_) return;
         std::vector<uint8_t> bytes = std::move(it->second);
         pending_.erase(it);
         write_index_++;
+        lk.unlock();
+        std::exception_ptr err;
         try {
             out_->write_all(bytes.data(), bytes.size());
         } catch (...) {
-            if (!sticky_) sticky_ = std::current_exception();
+            err = std::current_exception();
+        }
+        lk.lock();
+        if (err) {
+            if (!sticky_) sticky_ = err;
             return;
         }
     }
-    (void)lk;
+}
+
+// A pool of up to max_in_flight workers takes dispatched chunks in order, as
+// rayon's pool runs the spawned closures (parallel.rs:101-104).
+void ParallelChunkedWriter::worker() {
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+        cv_jobs_.wait(lk, [&] { return stop_ || !jobs_.empty(); });
+        if (jobs_.empty()) return;  // stop_ and drained
+        auto job = std::move(jobs_.front());
+        jobs_.pop_front();
+        lk.unlock();
+        std::vector<uint8_t> res;
+        std::exception_ptr err;
+        try {
+            res = fn_(job.second->data(), job.second->size());
+        } catch (...) {
+            err = std::current_exception();
+        }
+        job.second.reset();
+        lk.lock();
+        if (err) {
+            if (!sticky_) sticky_ = err;
+        } else {
+            pending_.emplace(job.first, std::move(res));
+        }
+        in_flight_--;
+        cv_.notify_all();
+    }
 }
 
 void ParallelChunkedWriter::dispatch_current() {
@@ -56,44 +92,17 @@ void ParallelChunkedWriter::dispatch_current() {
     auto bytes = std::make_shared<std::vector<uint8_t>>(std::move(current_));
     current_ = std::vector<uint8_t>();
     current_.reserve(chunk_size_);
-    {
-        std::lock_guard<std::mutex> lk(mu_);
-        in_flight_++;
-    }
-    // one worker thread per in-flight chunk, like rayon::spawn (parallel.rs:101-104)
-    threads_.emplace_back([this, idx, bytes] {
-        std::vector<uint8_t> res;
-        std::exception_ptr err;
-        try {
-            res = fn_(bytes->data(), bytes->size());
-        } catch (...) {
-            err = std::current_exception();
-        }
-        std::lock_guard<std::mutex> lk(mu_);
-        if (err) {
-            if (!sticky_) sticky_ = err;
-        } else {
-            pending_.emplace(idx, std::move(res));
-        }
-        in_flight_--;
-        cv_.notify_all();
-    });
     std::unique_lock<std::mutex> lk(mu_);
+    in_flight_++;
+    jobs_.emplace_back(idx, std::move(bytes));
+    if (workers_.size() < max_in_flight_ && workers_.size() < in_flight_) workers_.emplace_back([this] { worker(); });
+    cv_jobs_.notify_one();
     write_ready_locked(lk);
     while (in_flight_ >= max_in_flight_ && !sticky_) {
         cv_.wait(lk);
         write_ready_locked(lk);
     }
-    // reap finished threads
-    lk.unlock();
-    if (threads_.size() > 4 * max_in_flight_) {
-        std::unique_lock<std::mutex> lk2(mu_);
-        cv_.wait(lk2, [&] { return in_flight_ == 0; });
-        lk2.unlock();
-        for (auto &t : threads_) t.join();
-        threads_.clear();
-    }
-    check_sticky();
+    if (sticky_) std::rethrow_exception(sticky_);
 }
 
 void ParallelChunkedWriter::write_all(const uint8_t *data, size_t len) {
@@ -125,11 +134,17 @@ void ParallelChunkedWriter::finalize() {
     }
     {
         std::unique_lock<std::mutex> lk(mu_);
-        cv_.wait(lk, [&] { return in_flight_ == 0; });
+        for (;;) {
+            write_ready_locked(lk);
+            if (in_flight_ == 0) break;
+            cv_.wait(lk);
+        }
         write_ready_locked(lk);
+        stop_ = true;
     }
-    for (auto &t : threads_) t.join();
-    threads_.clear();
+    cv_jobs_.notify_all();
+    for (auto &t : workers_) t.join();
+    workers_.clear();
     try {
         out_->flush();
     } catch (...) {
@@ -159,9 +174,10 @@ class GpuGzipCompressor : public Compressor {
 
 }  // namespace
 
-std::shared_ptr<GpuBackend> GpuBackend::try_init() {
+std::shared_ptr<GpuBackend> GpuBackend::try_init(uint32_t lanes) {
     gpz_config cfg;
     gpz_config_default(&cfg);
+    if (lanes) cfg.lanes = lanes;
     gpz_ctx *ctx = nullptr;
     int rc = gpz_init(&cfg, &ctx);
     if (rc != GPZ_OK) throw Error(Error::Kind::Codec, std::string("gpu init failed: ") + gpz_last_error());
@@ -179,12 +195,13 @@ GpuBackend::~GpuBackend() {
 ChunkFn GpuBackend::gzip_chunk_fn() const {
     std::shared_ptr<const GpuBackend> self = shared_from_this();
     return [self](const uint8_t *data, size_t len) {
-        std::vector<uint8_t> out(gpz_compress_bound(self->ctx_, len));
+        // the bound is twice the chunk: no zero fill of it, one copy of what is used
+        const size_t cap = gpz_compress_bound(self->ctx_, len);
+        std::unique_ptr<uint8_t[]> buf(new uint8_t[cap]);
         size_t n = 0;
-        int rc = gpz_compress_chunk(self->ctx_, data, len, out.data(), out.size(), &n);
+        int rc = gpz_compress_chunk(self->ctx_, data, len, buf.get(), cap, &n);
         if (rc != GPZ_OK) throw Error(Error::Kind::Io, std::string("gpu compress failed: ") + gpz_last_error());
-        out.resize(n);
-        return out;
+        return std::vector<uint8_t>(buf.get(), buf.get() + n);
     };
 }
 
@@ -200,7 +217,7 @@ std::unique_ptr<Compressor> GpuBackend::compressor(Algorithm a, Level) const {
 const std::shared_ptr<GpuBackend> *LazyGpuBackend::try_get() const {
     std::call_once(once_, [this] {
         try {
-            inner_ = GpuBackend::try_init();
+            inner_ = GpuBackend::try_init(lanes_);
         } catch (...) {
             inner_.reset();
         }

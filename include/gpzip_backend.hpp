@@ -13,13 +13,20 @@
 //   LazyGpuBackend         codec-gpu/src/gpu.rs:28-96 (init once on first use;
 //                          try_get() == nullptr when no GPU, so callers fall
 //                          back to CPU as cli/src/hybrid.rs:128-136 does)
+//   CpuBackend,            codec-cpu/src/backend.rs:14-60, compress.rs:52-59
+//   GzipCompressor         (gzip only: one zlib gzip member per 2 MiB chunk)
+//   parallel_decompress,   codec-cpu/src/parallel_decompress.rs:35-113,
+//   GzipDecompressor       codec-cpu/src/decompress.rs:28-44 (the decode side)
+//   HybridBackend          cli/src/hybrid.rs:26-163 (`--backend auto`)
 //
 // Errors are thrown as gpzip::Error with the reference's error kinds.
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <condition_variable>
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
@@ -70,18 +77,27 @@ class ParallelChunkedWriter : public Writer {
     void dispatch_current();
     void write_ready_locked(std::unique_lock<std::mutex> &lk);
     void check_sticky();
+    void worker();
     std::unique_ptr<Writer> out_;
     size_t chunk_size_, max_in_flight_;
     ChunkFn fn_;
     std::vector<uint8_t> current_;
     uint32_t next_index_ = 0, write_index_ = 0;
-    size_t in_flight_ = 0;
+    size_t in_flight_ = 0;  // dispatched, result not yet posted
     std::map<uint32_t, std::vector<uint8_t>> pending_;
+    std::deque<std::pair<uint32_t, std::shared_ptr<std::vector<uint8_t>>>> jobs_;
     std::mutex mu_;
-    std::condition_variable cv_;
-    std::vector<std::thread> threads_;
+    std::condition_variable cv_, cv_jobs_;
+    std::vector<std::thread> workers_;
     std::exception_ptr sticky_;
-    bool finalized_ = false;
+    bool finalized_ = false, stop_ = false;
+};
+
+// Input source (Box<dyn Read + Send> in the reference); read() returns 0 at the end.
+class Reader {
+  public:
+    virtual ~Reader() = default;
+    virtual size_t read(uint8_t *buf, size_t cap) = 0;  // throws Error(Io)
 };
 
 class Compressor {
@@ -91,18 +107,68 @@ class Compressor {
     virtual std::unique_ptr<Writer> wrap_writer(std::unique_ptr<Writer> w) = 0;
 };
 
+class Decompressor {
+  public:
+    virtual ~Decompressor() = default;
+    virtual Algorithm algorithm() const = 0;
+    virtual std::unique_ptr<Reader> wrap_reader(std::unique_ptr<Reader> r) = 0;
+};
+
 class CodecBackend {
   public:
     virtual ~CodecBackend() = default;
     virtual const char *name() const = 0;
     virtual Capability supports(Algorithm a) const = 0;
     virtual std::unique_ptr<Compressor> compressor(Algorithm a, Level level) const = 0;
+    // compress-only backends (the GPU one, codec-gpu/src/gpu.rs:237-242)
+    virtual std::unique_ptr<Decompressor> decompressor(Algorithm) const {
+        throw Error(Error::Kind::DecompressionUnsupported, std::string("backend `") + name() + "` cannot decompress");
+    }
+};
+
+// ---- CPU side (host cores, system zlib) -------------------------------------
+
+// One gzip member of a chunk at `level` (flate2 GzEncoder's fixed header).
+std::vector<uint8_t> cpu_gzip_member(const uint8_t *data, size_t len, Level level);
+
+struct GzipCompressor {
+    static ChunkFn chunk_fn(Level level);  // codec-cpu/src/compress.rs:52-59
+};
+
+class CpuBackend : public CodecBackend {
+  public:
+    static constexpr const char *NAME = "cpu";
+    static constexpr size_t DEFAULT_CHUNK_SIZE = 2u << 20;  // codec-cpu/src/backend.rs:13
+    // max_in_flight 0: one per hardware thread (num_cpus)
+    explicit CpuBackend(size_t chunk_size = DEFAULT_CHUNK_SIZE, size_t max_in_flight = 0);
+    size_t chunk_size() const { return chunk_size_; }
+    size_t max_in_flight() const { return max_in_flight_; }
+    const char *name() const override { return NAME; }
+    Capability supports(Algorithm a) const override;
+    std::unique_ptr<Compressor> compressor(Algorithm a, Level level) const override;
+    std::unique_ptr<Decompressor> decompressor(Algorithm a) const override;
+
+  private:
+    size_t chunk_size_, max_in_flight_;
+};
+
+// Offsets of every fixed 10-byte gzip member header (1f 8b 08 00 00000000 00 ff).
+std::vector<size_t> find_member_starts(const uint8_t *data, size_t len);
+// Decode a whole gzip stream: members in parallel on `threads` workers (0: one
+// per hardware thread) when the scan finds two or more, else (or when any
+// member fails) a serial multi-member decode.  Throws Error(Io) on a corrupt stream.
+std::vector<uint8_t> parallel_decompress(const uint8_t *data, size_t len, unsigned threads = 0);
+
+class GzipDecompressor : public Decompressor {
+  public:
+    Algorithm algorithm() const override { return Algorithm::Gzip; }
+    std::unique_ptr<Reader> wrap_reader(std::unique_ptr<Reader> r) override;  // buffers the whole input
 };
 
 class GpuBackend : public CodecBackend, public std::enable_shared_from_this<GpuBackend> {
   public:
     static constexpr const char *NAME = "gpu";
-    static std::shared_ptr<GpuBackend> try_init();  // throws Error(Codec)
+    static std::shared_ptr<GpuBackend> try_init(uint32_t lanes = 0);  // throws Error(Codec)
     ~GpuBackend() override;
     size_t chunk_size() const { return chunk_size_; }
     size_t max_in_flight() const { return max_in_flight_; }
@@ -122,14 +188,50 @@ class GpuBackend : public CodecBackend, public std::enable_shared_from_this<GpuB
 class LazyGpuBackend : public CodecBackend {
   public:
     LazyGpuBackend() = default;
+    // lanes: concurrent GPU pipelines of the context (0: the library default)
+    explicit LazyGpuBackend(uint32_t lanes) : lanes_(lanes) {}
     const std::shared_ptr<GpuBackend> *try_get() const;  // nullptr when no GPU
     const char *name() const override { return GpuBackend::NAME; }
     Capability supports(Algorithm a) const override;
     std::unique_ptr<Compressor> compressor(Algorithm a, Level level) const override;
 
   private:
+    uint32_t lanes_ = 0;
     mutable std::once_flag once_;
     mutable std::shared_ptr<GpuBackend> inner_;
+};
+
+// ---- `--backend auto` (cli/src/hybrid.rs) -----------------------------------
+
+struct HybridStats {
+    std::atomic<uint64_t> gpu_chunks{0}, cpu_chunks{0};
+};
+
+struct HybridTuning {
+    size_t gpu_workers = 1;    // GPU permits (hybrid.rs:72)
+    size_t warmup_chunks = 4;  // first chunks always on the CPU (hybrid.rs:118)
+    // with no permit free, wait this long for one before the chunk goes to
+    // the CPU (the reference falls back at once: 0)
+    uint32_t gpu_wait_ms = 0;
+    std::shared_ptr<HybridStats> stats;  // optional counters
+    static HybridTuning reference() { return HybridTuning{}; }
+    static HybridTuning retuned();
+};
+
+class HybridBackend : public CodecBackend {
+  public:
+    static constexpr const char *NAME = "hybrid";
+    HybridBackend(std::shared_ptr<CpuBackend> cpu, std::shared_ptr<LazyGpuBackend> gpu,
+                  HybridTuning t = HybridTuning::reference());
+    const char *name() const override { return NAME; }
+    Capability supports(Algorithm a) const override;
+    std::unique_ptr<Compressor> compressor(Algorithm a, Level level) const override;
+    std::unique_ptr<Decompressor> decompressor(Algorithm a) const override;
+
+  private:
+    std::shared_ptr<CpuBackend> cpu_;
+    std::shared_ptr<LazyGpuBackend> gpu_;
+    HybridTuning t_;
 };
 
 }  // namespace gpzip

@@ -134,6 +134,20 @@ int gpz_encode_block(gpz_ctx *ctx, const uint32_t *walked, size_t n, uint8_t *ou
 int gpz_gzip_member(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out, size_t out_cap,
                     size_t *out_len);
 
+/* ---- decode side (host cores) --------------------------------------------
+ * The reference decodes on the CPU: gpzip x scans a gzip stream for the fixed
+ * 10-byte member header and inflates the members in parallel, falling back
+ * to a serial multi-member decode (codec-cpu/src/parallel_decompress.rs:35-113,
+ * codec-cpu/src/decompress.rs:28-44).  Host code on system zlib; no GPU. */
+
+/* Offsets of the fixed gzip member headers in [in, in + len).  *count gets
+ * the number found; GPZ_E_NOSPACE when more than cap. */
+int gpz_gzip_member_starts(const uint8_t *in, size_t len, uint64_t *starts, size_t cap, size_t *count);
+/* Decode a gzip stream (any number of members) into out.  threads = 0: one
+ * per hardware thread.  GPZ_E_NOSPACE with *out_len = the decoded size when
+ * out_cap is too small; GPZ_E_INVAL for a corrupt stream. */
+int gpz_gunzip(const uint8_t *in, size_t len, uint8_t *out, size_t out_cap, size_t *out_len, unsigned threads);
+
 /* ---- batched form (host buffers) ---------------------------------------- */
 
 typedef struct gpz_desc {

@@ -2,6 +2,11 @@
 // reference's scheduler tests (codec-cpu/src/parallel.rs:231-424).
 //   test_backend cpu        -- ParallelChunkedWriter with an identity framer, no GPU
 //   test_backend gpu FILE   -- GpuBackend compressor over FILE, output to stdout
+//   test_backend hybrid FILE GPU_WORKERS WARMUP [GPU_WAIT_MS]
+//                           -- HybridBackend (cli/src/hybrid.rs) over FILE,
+//                              stream to stdout, "gpu_chunks N cpu_chunks M
+//                              seconds S" to stderr
+//   test_backend decode FILE -- GzipDecompressor over FILE, output to stdout
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -158,6 +163,80 @@ static void lazy_backend_without_gpu() {
     }
 }
 
+struct VecReader : Reader {
+    std::vector<uint8_t> b;
+    size_t pos = 0;
+    size_t read(uint8_t *buf, size_t cap) override {
+        size_t n = std::min(cap, b.size() - pos);
+        memcpy(buf, b.data() + pos, n);
+        pos += n;
+        return n;
+    }
+};
+
+static std::vector<uint8_t> read_all(Reader &r) {
+    std::vector<uint8_t> out;
+    uint8_t tmp[4096];
+    size_t n;
+    while ((n = r.read(tmp, sizeof tmp)) > 0) out.insert(out.end(), tmp, tmp + n);
+    return out;
+}
+
+static std::vector<uint8_t> pattern(size_t n, uint32_t seed) {
+    std::vector<uint8_t> d(n);
+    uint32_t x = seed;
+    for (size_t i = 0; i < n; i++) {
+        x = x * 1664525u + 1013904223u;
+        d[i] = (i / 97) % 3 ? (uint8_t)("abcdefgh"[(x >> 24) & 7]) : (uint8_t)(x >> 24);
+    }
+    return d;
+}
+
+// CpuBackend compress (codec-cpu/src/compress.rs) -> GzipDecompressor
+// (codec-cpu/src/decompress.rs): the round trip, one member per 2 MiB chunk.
+static void cpu_codec_round_trip() {
+    CpuBackend cpu(1 << 20, 3);
+    CHECK(cpu.supports(Algorithm::Gzip) == Capability::Both);
+    const std::vector<uint8_t> data = pattern((5 << 20) + 777, 7);
+    auto sink = std::make_unique<Sink>();
+    auto buf = sink->buf;
+    {
+        auto w = cpu.compressor(Algorithm::Gzip, Level{5})->wrap_writer(std::move(sink));
+        w->write_all(data.data(), data.size());
+    }
+    CHECK(find_member_starts(buf->data(), buf->size()).size() == 6);
+    auto rd = std::make_unique<VecReader>();
+    rd->b = *buf;
+    auto r = cpu.decompressor(Algorithm::Gzip)->wrap_reader(std::move(rd));
+    CHECK(read_all(*r) == data);
+    CHECK(parallel_decompress(buf->data(), buf->size(), 1) == data);
+    bool threw = false;
+    try {
+        LazyGpuBackend().decompressor(Algorithm::Gzip);
+    } catch (const Error &e) {
+        threw = e.kind == Error::Kind::DecompressionUnsupported;
+    }
+    CHECK(threw);
+}
+
+// `--backend auto` on a host without a GPU: every chunk on the CPU path.
+static void hybrid_without_gpu() {
+    auto stats = std::make_shared<HybridStats>();
+    HybridTuning t = HybridTuning::retuned();
+    t.stats = stats;
+    HybridBackend hy(std::make_shared<CpuBackend>(1 << 20, 2), std::make_shared<LazyGpuBackend>(), t);
+    CHECK(std::string(hy.name()) == "hybrid");
+    const std::vector<uint8_t> data = pattern((7 << 20) + 5, 9);
+    auto sink = std::make_unique<Sink>();
+    auto buf = sink->buf;
+    {
+        auto w = hy.compressor(Algorithm::Gzip, Level{5})->wrap_writer(std::move(sink));
+        w->write_all(data.data(), data.size());
+    }
+    CHECK(parallel_decompress(buf->data(), buf->size()) == data);
+    if (!LazyGpuBackend().try_get()) CHECK(stats->gpu_chunks == 0 && stats->cpu_chunks == 8);
+}
+
 struct FileSink : Writer {
     FILE *f;
     explicit FileSink(FILE *f) : f(f) {}
@@ -173,7 +252,52 @@ int main(int argc, char **argv) {
         backpressure();
         sticky_error();
         lazy_backend_without_gpu();
+        cpu_codec_round_trip();
+        hybrid_without_gpu();
         printf("cpu ok\n");
+        return 0;
+    }
+    if (argc >= 5 && !strcmp(argv[1], "hybrid")) {
+        FILE *in = fopen(argv[2], "rb");
+        CHECK(in);
+        std::vector<uint8_t> data;
+        uint8_t tmp[1 << 16];
+        size_t n;
+        while ((n = fread(tmp, 1, sizeof tmp, in)) > 0) data.insert(data.end(), tmp, tmp + n);
+        fclose(in);
+        auto stats = std::make_shared<HybridStats>();
+        HybridTuning t;
+        t.gpu_workers = (size_t)atoi(argv[3]);
+        t.warmup_chunks = (size_t)atoi(argv[4]);
+        t.gpu_wait_ms = argc >= 6 ? (uint32_t)atoi(argv[5]) : 0;
+        t.stats = stats;
+        auto lazy = std::make_shared<LazyGpuBackend>((uint32_t)std::max<size_t>(4, t.gpu_workers));
+        CHECK(lazy->try_get() != nullptr);  // initialised before the clock starts
+        HybridBackend hy(std::make_shared<CpuBackend>(), lazy, t);
+        auto sink = std::make_unique<Sink>();
+        auto buf = sink->buf;
+        auto t0 = std::chrono::steady_clock::now();
+        {
+            auto w = hy.compressor(Algorithm::Gzip, Level{5})->wrap_writer(std::move(sink));
+            w->write_all(data.data(), data.size());
+        }
+        double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        fwrite(buf->data(), 1, buf->size(), stdout);
+        fprintf(stderr, "gpu_chunks %llu cpu_chunks %llu seconds %.6f\n", (unsigned long long)stats->gpu_chunks.load(),
+                (unsigned long long)stats->cpu_chunks.load(), sec);
+        return 0;
+    }
+    if (argc >= 3 && !strcmp(argv[1], "decode")) {
+        FILE *in = fopen(argv[2], "rb");
+        CHECK(in);
+        auto rd = std::make_unique<VecReader>();
+        uint8_t tmp[1 << 16];
+        size_t n;
+        while ((n = fread(tmp, 1, sizeof tmp, in)) > 0) rd->b.insert(rd->b.end(), tmp, tmp + n);
+        fclose(in);
+        auto r = CpuBackend().decompressor(Algorithm::Gzip)->wrap_reader(std::move(rd));
+        std::vector<uint8_t> out = read_all(*r);
+        fwrite(out.data(), 1, out.size(), stdout);
         return 0;
     }
     if (argc >= 3 && !strcmp(argv[1], "gpu")) {

@@ -600,3 +600,28 @@ def test_server_admission_blocks_at_budget(ctx):
     assert st["waited"] > 0, "no submission waited for pool space"
     for i, r in enumerate(reqs):
         assert results[i] == ctx.compress_chunk(r), f"request {i}"
+
+
+def test_gunzip_decodes_every_form(ctx):
+    """gpzip x's decoder (gpz_gunzip: member scan, parallel inflate) decodes
+    the plain, batched and served outputs back to their inputs."""
+    kinds = ["text", "binmix", "log"]
+    chunks = [gpzip_amd.synth(kinds[i % 3], s, 500 + i) for i, s in enumerate([0, 1, 4097, 131072, 3 << 20, 2 << 20])]
+    say("gunzip plain / batched / served")
+    for c, b in zip(chunks, ctx.batch(chunks)):
+        assert gpzip_amd.gunzip(b) == c.tobytes()
+        assert gpzip_amd.gunzip(ctx.compress_chunk(c)) == c.tobytes()
+    srv = gpzip_amd.Server(ctx, streams=2, budget_bytes=2 << 30)
+    try:
+        for i, c in enumerate(chunks):
+            srv.submit(c, i)
+        got = {}
+        while len(got) < len(chunks):
+            tag, data, _ = srv.next()
+            got[tag] = data
+    finally:
+        srv.stop()
+    for i, c in enumerate(chunks):
+        assert gpzip_amd.gunzip(got[i]) == c.tobytes()
+    whole = b"".join(got[i] for i in range(len(chunks)))
+    assert gpzip_amd.gunzip(whole) == b"".join(c.tobytes() for c in chunks)
