@@ -18,6 +18,7 @@ SYNTH_LIB_PATH = os.path.join(HERE, "libgpzip_synth.so")
 
 GPZ_OK, GPZ_E_NODEV, GPZ_E_OOM, GPZ_E_INVAL, GPZ_E_DEVICE, GPZ_E_NOSPACE, GPZ_E_BUSY, GPZ_E_AGAIN = range(8)
 GPZ_PLAN_GRAPH, GPZ_PLAN_PERSISTENT, GPZ_PLAN_TIMED = 1, 2, 4
+GPZ_CFG_RATIO = 1
 
 # Every entry point include/gpzip_hip.h declares.
 ABI_SYMBOLS = [
@@ -167,13 +168,15 @@ def device_count() -> int:
 class Context:
     """One GPU context (gpz_ctx)."""
 
-    def __init__(self, device: int = -1, sub_chunk: int = 128 * 1024, lanes: int = 4):
+    def __init__(self, device: int = -1, sub_chunk: int = 128 * 1024, lanes: int = 4, flags: int = 0):
+        """flags: GPZ_CFG_RATIO for the opt-in ratio mode (outside reference parity)."""
         L = load_library()
         cfg = Config()
         L.gpz_config_default(C.byref(cfg))
         cfg.device = device
         cfg.sub_chunk = sub_chunk
         cfg.lanes = lanes
+        cfg.flags = flags
         self.cfg = cfg
         h = C.c_void_p()
         _check(L.gpz_init(C.byref(cfg), C.byref(h)))
@@ -219,12 +222,15 @@ class Context:
         _check(self.L.gpz_match_find(self.h, _ptr(a), a.size, _ptr(tok)))
         return tok
 
-    def walk(self, tokens, data=None, strict: bool = False) -> np.ndarray:
+    def walk(self, tokens, data=None, strict: bool = False, rule: int | None = None) -> np.ndarray:
+        """rule: 0 the reference's lazy walk, 1 strict, 2 the ratio mode's
+        lookahead walk (default: 1 if strict else 0)."""
         t = np.ascontiguousarray(tokens, dtype=np.uint32)
         a = _u8(data) if data is not None else None
         out = np.zeros(max(t.size, 1), np.uint32)
         n = C.c_size_t()
-        _check(self.L.gpz_walk(self.h, _ptr(a), _ptr(t), t.size, int(strict), _ptr(out), C.byref(n)))
+        r = int(strict) if rule is None else rule
+        _check(self.L.gpz_walk(self.h, _ptr(a), _ptr(t), t.size, r, _ptr(out), C.byref(n)))
         return out[: n.value].copy()
 
     def encode_block(self, walked) -> bytes:

@@ -20,8 +20,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["gpz_kernels.hip", "gpz_runtime.hip", "gpz_batch.hip", "gpz_server.hip", "gpz_backend.hip"]
-# host-only C++ (the CPU codec and decode side, the hybrid scheduler), built with g++
-CPP_SOURCES = ["gpz_cpu.cpp", "gpz_hybrid.cpp"]
+# host-only C++ (the CPU codec and decode side, the hybrid scheduler, the tar
+# front-end), built with g++
+CPP_SOURCES = ["gpz_cpu.cpp", "gpz_hybrid.cpp", "gpz_archive.cpp"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

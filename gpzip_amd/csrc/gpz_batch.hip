@@ -101,7 +101,7 @@ int lane_step_resident(gpz_ctx *ctx, Lane *ln, const Plan &pl, const void *d_in)
     Work w = ln->wsp.work;
     w.in = (const uint8_t *)d_in;
     w.gzip = 1;
-    w.strict = 0;
+    w.strict = ctx_walk_rule(ctx);
     Grids g = grids_for(ln->wsp.caps, ctx_cus(ctx));
     apply_tune(g, ctx_tune(ctx), ln->wsp.caps.G, ln->wsp.caps.M);  // capacity: stable, so the graph is kept
     if (!ln->graph || ln->graph_gen != ln->wsp.generation || ln->graph_in != d_in || !(ln->graph_grids == g)) {
@@ -435,7 +435,7 @@ int gpz_plan_run(gpz_plan *p, const void *d_in, void *d_out) {
     w.in = (const uint8_t *)d_in;
     w.out = (uint8_t *)d_out;
     w.gzip = 1;
-    w.strict = 0;
+    w.strict = ctx_walk_rule(p->ctx);
     if (p->flags & GPZ_PLAN_GRAPH) {
         if (!ln.graph || p->bound_in != d_in || p->bound_out != d_out) {
             int rc = capture_graph(w, p->grids, ln.st, &ln.graph);

@@ -69,7 +69,8 @@ struct Work {
     uint64_t *chunk_len;        // [C]
     uint64_t *total;            // [1]
     uint8_t *out;               // output bytes (4-byte aligned)
-    uint32_t strict;            // walk rule: 0 lazy (production), 1 strict
+    uint32_t strict;            // walk rule: 0 lazy (production), 1 strict, 2 lazy with a two-token
+                                // lookahead (the opt-in ratio mode)
     uint32_t gzip;              // 1: gzip members, 0: raw DEFLATE (encode_block)
     uint32_t window;            // 32768
     uint32_t pad;

@@ -254,14 +254,27 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Successor of segment position x.  next_known: tokL[L] holds the next
-// segment's first token.  absorb_last: the last position's successor depends
-// on the next segment and is left to k_chain (self-loop).
+// Successor of segment position x.  next_known: tokL[L] (and tokL[L+1] for
+// rule 2) hold the next segment's first tokens, 0 past the member end.
+// absorb_last: the last position's successor depends on the next segment and
+// is left to k_chain (self-loop).  Rule 2 is the opt-in ratio mode (no
+// reference counterpart): a match is demoted when the next position holds a
+// longer match or the one after it a match longer by two or more; it is only
+// walked with the next tokens known.
 __device__ __forceinline__ uint32_t succ_of(const uint32_t *tokL, uint32_t x, uint32_t L, bool next_known,
                                             uint32_t strict, bool absorb_last, bool &dem) {
     dem = false;
     const uint32_t t = tokL[x], len = t >> 16;
     if (len == 0) return x + 1;
+    if (strict == 2) {
+        const uint32_t n1 = (x + 1 < L || next_known) ? tokL[x + 1] >> 16 : 0;
+        const uint32_t n2 = (x + 2 < L || next_known) ? tokL[x + 2] >> 16 : 0;
+        if (n1 > len || n2 > len + 1) {
+            dem = true;
+            return x + 1;
+        }
+        return x + len;
+    }
     if (strict) return x + len;
     if (x + 1 < L || next_known) {
         const uint32_t nl = tokL[x + 1] >> 16;
@@ -484,10 +497,12 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
 // The walk summary of segment g from its tokens in LDS (tokL[0..L)); 1024
 // threads.  P is a ping-pong pair of 4096 words exit | count << 16.  Ends
 // before the caller's closing barrier.
+// next_known: tokL[L], tokL[L+1] hold the next segment's first tokens (0 past
+// the member end), so no position is absorbed.
 __device__ __forceinline__ void seg_summary(const Work &w, uint32_t g, uint32_t L, bool last_seg, bool any_match,
-                                            const uint32_t *tokL, uint32_t *P0, uint32_t *P1) {
+                                            const uint32_t *tokL, uint32_t *P0, uint32_t *P1, bool next_known = false) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const bool absorb_last = !w.strict && !last_seg;
+    const bool absorb_last = w.strict != 1 && !last_seg && !next_known;
     uint32_t *out = w.wsum + (uint64_t)g * SPAN;
     // the last position is a self-loop only when it holds a match
     const uint32_t absorb = (absorb_last && (tokL[L - 1] >> 16) != 0) ? L - 1 : 0xFFFFu;
@@ -504,7 +519,7 @@ __device__ __forceinline__ void seg_summary(const Work &w, uint32_t g, uint32_t 
         const uint32_t x = bs + lane;
         const bool valid = x < L;
         bool dem;
-        const uint32_t J = valid ? succ_of(tokL, x, L, false, w.strict, absorb_last, dem) : 0;
+        const uint32_t J = valid ? succ_of(tokL, x, L, next_known, w.strict, absorb_last, dem) : 0;
         uint32_t c;
         const uint32_t ex = block_walk_count(J, bs, L, valid, valid && x == absorb, c);
         if (valid) P0[x] = ex | (c << 16);
@@ -879,22 +894,25 @@ constexpr int KW_T = 1024;
 
 
 __global__ __launch_bounds__(KW_T) void k_walk_summary(Work w) {
-    __shared__ uint32_t tokL[SEG];
+    __shared__ uint32_t tokL[SEG + 2];
     __shared__ uint32_t P[2][SEG];
     const uint32_t tid = threadIdx.x;
     const uint32_t G = uni(w.counts->nseg);
+    // rule 2 reads the next segment's first two tokens (all tokens exist when
+    // this kernel runs), so no position is left to k_chain
+    const bool lookahead = w.strict == 2;
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
         const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
         int anym = 0;
-        for (uint32_t x = tid; x < L; x += KW_T) {
-            const uint32_t t = w.tok[md.pos_base + S + x];
+        for (uint32_t x = tid; x < L + 2; x += KW_T) {
+            const uint32_t t = S + x < n ? w.tok[md.pos_base + S + x] : 0;
             tokL[x] = t;
-            anym |= (t >> 16) != 0;
+            anym |= x < L && (t >> 16) != 0;
         }
         const int any_match = __syncthreads_or(anym);
-        seg_summary(w, g, L, E == n, any_match != 0, tokL, P[0], P[1]);
+        seg_summary(w, g, L, E == n, any_match != 0, tokL, P[0], P[1], lookahead);
         __syncthreads();
     }
 }
@@ -991,12 +1009,13 @@ constexpr int K3_T = 1024;
 // (lgkmcnt) one, which leaves them outstanding.
 struct SegPrefetch {
     uint4 q;
-    uint32_t raw, tn;
+    uint32_t raw, tn, tn2;
 };
 __device__ __forceinline__ void tok_prefetch(const Work &w, uint32_t G, uint32_t g, SegPrefetch &f) {
     f.q = make_uint4(0, 0, 0, 0);
     f.raw = 0;
     f.tn = 0;
+    f.tn2 = 0;
     if (g >= G) return;
     const MemberDesc md = load_member(w.mem, sload_u32(w.seg_mem, g));
     const uint32_t S = (g - md.seg_base) << SEG_LOG2, E = min(S + SEG, md.n), L = E - S;
@@ -1016,6 +1035,7 @@ __device__ __forceinline__ void tok_prefetch(const Work &w, uint32_t G, uint32_t
         if (x + 2 < L) f.raw |= (uint32_t)ib[x + 2] << 16;
     }
     if (threadIdx.x == 0 && E < md.n) f.tn = tg[L];
+    if (threadIdx.x == 0 && E + 1 < md.n) f.tn2 = tg[L + 1];
 }
 
 __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
@@ -1046,7 +1066,10 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
         const uint32_t tokbase = sload_u32(w.seg_tokbase, g);
         reinterpret_cast<uint4 *>(tokL)[tid] = pf.q;
         rawL[tid] = pf.raw;
-        if (tid == 0 && L == SEG) tokL[SEG] = pf.tn;  // (a shorter segment is its member's last)
+        if (tid == 0 && L == SEG) {  // (a shorter segment is its member's last)
+            tokL[SEG] = pf.tn;
+            tokL[SEG + 1] = pf.tn2;
+        }
         // the stores above wait for this segment's loads only if the next
         // segment's are not yet issued: keep them apart
         asm volatile("" ::: "memory");

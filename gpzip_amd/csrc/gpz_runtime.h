@@ -120,6 +120,9 @@ struct Lane {
 int ctx_device(gpz_ctx *c);
 int ctx_cus(gpz_ctx *c);
 uint32_t ctx_sub_chunk(gpz_ctx *c);
+// walk rule of the context's compress paths: 0 the reference's lazy rule,
+// 2 the opt-in ratio mode (GPZ_CFG_RATIO)
+uint32_t ctx_walk_rule(gpz_ctx *c);
 const gpz_tune &ctx_tune(gpz_ctx *c);
 Lane *ctx_acquire(gpz_ctx *c);
 void ctx_release(gpz_ctx *c, Lane *l);

@@ -329,8 +329,16 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
         default:  // MODE_FULL
             launch_seg_table(w, g.k0, st);
             if (ev) hipEventRecord(ev[5], st);
-            launch_match_find(w, g.k1, 1, g.k1v, st);  // with the walk summary
-            if (ev) hipEventRecord(ev[1], st);
+            if (w.strict == 2) {
+                // ratio mode: the walk rule reads two tokens ahead, so the
+                // summary runs once every token exists
+                launch_match_find(w, g.k1, 0, g.k1v, st);
+                if (ev) hipEventRecord(ev[1], st);
+                launch_walk_summary(w, g.k1, st);
+            } else {
+                launch_match_find(w, g.k1, 1, g.k1v, st);  // with the walk summary
+                if (ev) hipEventRecord(ev[1], st);
+            }
             launch_chain(w, g.chain, st);
             launch_walk_emit(w, g.k3, st);
             launch_huffman(w, g.huff, st);
@@ -433,6 +441,7 @@ namespace gpz {
 int ctx_device(gpz_ctx *c) { return c->device; }
 int ctx_cus(gpz_ctx *c) { return c->n_cu; }
 uint32_t ctx_sub_chunk(gpz_ctx *c) { return c->cfg.sub_chunk; }
+uint32_t ctx_walk_rule(gpz_ctx *c) { return (c->cfg.flags & GPZ_CFG_RATIO) ? 2u : 0u; }
 const gpz_tune &ctx_tune(gpz_ctx *c) { return c->tune; }
 Lane *ctx_acquire(gpz_ctx *c) { return c->acquire(); }
 void ctx_release(gpz_ctx *c, Lane *l) { c->release(l); }
@@ -561,6 +570,7 @@ int gpz_init(const gpz_config *cfg_in, gpz_ctx **out) {
         gpz_config_default(&cfg);
     if (cfg.sub_chunk == 0 || cfg.sub_chunk > (16u << 20)) return fail(GPZ_E_INVAL, "sub_chunk out of range");
     if (cfg.window != 32768) return fail(GPZ_E_INVAL, "only the 32 KiB DEFLATE window is supported");
+    if (cfg.flags & ~(uint32_t)GPZ_CFG_RATIO) return fail(GPZ_E_INVAL, "unknown gpz_config flag");
     if (cfg.lanes == 0) cfg.lanes = 1;
     if (cfg.lanes > 32) cfg.lanes = 32;
     int n = 0;
@@ -634,10 +644,11 @@ int gpz_compress_chunk(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out
     int rc = build_plan(pl, &l, nullptr, 1, ctx->cfg.sub_chunk, 1);
     if (rc) return rc;
     Lane *ln = ctx->acquire();
-    ln->strict = 0;
+    ln->strict = ctx_walk_rule(ctx);
     rc = lane_run(ctx, ln, pl, MODE_FULL, in, &l);
     if (!rc) rc = lane_fetch_chunk(ln, 0, out, out_cap, out_len);
     if (rc) hipStreamSynchronize(ln->st);
+    ln->strict = 0;
     ctx->release(ln);
     return rc;
 }
@@ -673,7 +684,7 @@ int gpz_compress_chunk_noalloc(gpz_ctx *ctx, const uint8_t *in, size_t len, uint
     Work w = work_at(L, base);
     w.in = base + in_at;
     w.gzip = 1;
-    w.strict = 0;
+    w.strict = ctx_walk_rule(ctx);
     // no array needs to start at zero: every kernel writes what it later
     // reads (k_chain clears the histograms), as on reused lane workspaces
     if (len) HIPCHK(hipMemcpyAsync(base + in_at, in, len, hipMemcpyDefault, st));
@@ -703,10 +714,11 @@ int gpz_gzip_member(gpz_ctx *ctx, const uint8_t *in, size_t len, uint8_t *out, s
     int rc = build_plan(pl, &l, nullptr, 1, (uint32_t)std::max<size_t>(len, 1), 1);
     if (rc) return rc;
     Lane *ln = ctx->acquire();
-    ln->strict = 0;
+    ln->strict = ctx_walk_rule(ctx);
     rc = lane_run(ctx, ln, pl, MODE_FULL, in, &l);
     if (!rc) rc = lane_fetch_chunk(ln, 0, out, out_cap, out_len);
     if (rc) hipStreamSynchronize(ln->st);
+    ln->strict = 0;
     ctx->release(ln);
     return rc;
 }
@@ -742,7 +754,8 @@ int gpz_match_find(gpz_ctx *ctx, const uint8_t *in, size_t len, uint32_t *tokens
 int gpz_walk(gpz_ctx *ctx, const uint8_t *in, const uint32_t *per_pos, size_t len, int strict,
              uint32_t *walked, size_t *n_walked) {
     clear_error();
-    if (!ctx || !n_walked || (len && (!per_pos || !walked || (!strict && !in)))) return fail(GPZ_E_INVAL, "bad argument");
+    if (!ctx || !n_walked || strict < 0 || strict > 2 || (len && (!per_pos || !walked || (strict != 1 && !in))))
+        return fail(GPZ_E_INVAL, "bad argument");
     *n_walked = 0;
     if (len == 0) return GPZ_OK;
     if (len > (16u << 20)) return fail(GPZ_E_INVAL, "more than 16 Mi positions");
@@ -753,7 +766,7 @@ int gpz_walk(gpz_ctx *ctx, const uint8_t *in, const uint32_t *per_pos, size_t le
     int rc = build_plan(pl, &l, nullptr, 1, (uint32_t)len, 1);
     if (rc) return rc;
     Lane *ln = ctx->acquire();
-    ln->strict = strict ? 1 : 0;
+    ln->strict = (uint32_t)strict;
     do {
         Caps want{pl.P, pl.G, pl.M, pl.C, pl.out_cap};
         if ((rc = ln->wsp.reserve(want))) break;

@@ -18,6 +18,9 @@
 //   parallel_decompress,   codec-cpu/src/parallel_decompress.rs:35-113,
 //   GzipDecompressor       codec-cpu/src/decompress.rs:28-44 (the decode side)
 //   HybridBackend          cli/src/hybrid.rs:26-163 (`--backend auto`)
+//   TarBuilder,            core/src/archive/pack.rs:103-176 (`gpzip a x.tar.gz`),
+//   BatchedChunkedWriter,  with the chunks of the stream submitted to the GPU
+//   pack_tar_gz            as batches of (offset, len) descriptors
 //
 // Errors are thrown as gpzip::Error with the reference's error kinds.
 #pragma once
@@ -37,6 +40,7 @@
 #include <vector>
 
 struct gpz_ctx;
+struct gpz_batch;
 
 namespace gpzip {
 
@@ -200,6 +204,64 @@ class LazyGpuBackend : public CodecBackend {
     mutable std::once_flag once_;
     mutable std::shared_ptr<GpuBackend> inner_;
 };
+
+// ---- `gpzip a` onto the batched GPU form --------------------------------------
+
+// ParallelChunkedWriter's contract -- chunk_size chunks, members in chunk
+// order, flush() does not cut a chunk, sticky first error -- with every
+// batch_chunks chunks submitted as one device batch (gpz_batch_submit, one
+// graph replay); up to max_batches batches in flight.  The members equal the
+// per-chunk GPU writer's for the same chunk size.
+class BatchedChunkedWriter : public Writer {
+  public:
+    BatchedChunkedWriter(std::unique_ptr<Writer> out, gpz_ctx *ctx, size_t chunk_size = 2u << 20,
+                         size_t batch_chunks = 64, size_t max_batches = 2);
+    ~BatchedChunkedWriter() override;
+    void write_all(const uint8_t *data, size_t len) override;
+    void flush() override;
+    void finalize();  // idempotent; throws the sticky error
+    size_t batches_submitted() const { return batches_submitted_; }
+    size_t chunks_submitted() const { return chunks_submitted_; }
+
+  private:
+    void submit();
+    void drain_one();
+    void check_sticky();
+    std::unique_ptr<Writer> out_;
+    gpz_ctx *ctx_;
+    size_t chunk_size_, batch_chunks_, max_batches_;
+    std::vector<uint8_t> buf_;
+    std::deque<gpz_batch *> inflight_;
+    std::deque<size_t> counts_;
+    std::string sticky_;
+    bool finalized_ = false;
+    size_t batches_submitted_ = 0, chunks_submitted_ = 0;
+};
+
+// The tar stream of pack_tar (GNU headers; directories, then their entries in
+// name order; symlinks stored, not followed; two zero blocks at the end).
+class TarBuilder {
+  public:
+    explicit TarBuilder(std::unique_ptr<Writer> out);
+    ~TarBuilder();  // finish(), errors swallowed
+    void append_path(const std::string &src, const std::string &arc_path);
+    void append_data(const std::string &arc_path, const uint8_t *data, size_t len, uint32_t mode = 0644,
+                     uint64_t mtime = 0);
+    void finish();
+    Writer *inner() { return out_.get(); }
+
+  private:
+    void header(const std::string &path, char type, uint64_t size, uint32_t mode, uint64_t mtime,
+                const std::string &link);
+    static void finish_header(uint8_t *h);
+    void pad(uint64_t size);
+    std::unique_ptr<Writer> out_;
+    bool finished_ = false;
+};
+
+// `gpzip a archive.tar.gz inputs...` on the GPU's batched form.
+void pack_tar_gz(const std::string &archive, const std::vector<std::string> &inputs, gpz_ctx *ctx,
+                 size_t chunk_size = 2u << 20, size_t batch_chunks = 64);
 
 // ---- `--backend auto` (cli/src/hybrid.rs) -----------------------------------
 

@@ -65,9 +65,18 @@ typedef struct gpz_config {
     uint32_t window;       /* DEFLATE window; reference fixes 32 KiB (lz77_hash.rs:48) */
     uint32_t lanes;        /* concurrent per-chunk pipelines (streams) */
     uint32_t max_in_flight;/* advertised to ParallelChunkedWriter (gpu.rs:165) */
-    uint32_t flags;        /* reserved, 0 */
+    uint32_t flags;        /* GPZ_CFG_* bits, 0 by default */
     uint64_t lane_bytes;   /* per-lane input capacity reserved at init (0 = 2 MiB) */
 } gpz_config;
+
+/* GPZ_CFG_RATIO: the opt-in ratio mode, outside reference parity (no reference
+ * counterpart).  The walk demotes a match to a literal when the next position
+ * holds a longer match (the reference's lazy rule) or the position after it a
+ * match longer by two or more; 0.5-2 % smaller output on the synthetic
+ * corpora, same match finder.  Applies to every compress path of the context
+ * (plain, no-alloc, batched, plans, serving); gpz_walk takes the rule as
+ * strict = 2 in any context. */
+enum { GPZ_CFG_RATIO = 1u };
 
 /* Fill *cfg with the reference's defaults: 128 KiB members, 32 KiB window,
  * 16 in flight. */
@@ -121,7 +130,8 @@ int gpz_match_find(gpz_ctx *ctx, const uint8_t *in, size_t len, uint32_t *tokens
 
 /* Walk per-position tokens into the emitted token stream.  strict = 0: lazy
  * rule of greedy_walk (needs in for demoted literals); strict = 1: the
- * strict rule of host_strict_greedy (in may be NULL).  Token lengths must be
+ * strict rule of host_strict_greedy (in may be NULL); strict = 2: the ratio
+ * mode's lookahead rule (GPZ_CFG_RATIO; needs in).  Token lengths must be
  * 0..258.  walked must hold len entries; *n_walked receives the count. */
 int gpz_walk(gpz_ctx *ctx, const uint8_t *in, const uint32_t *per_pos, size_t len, int strict,
              uint32_t *walked, size_t *n_walked);

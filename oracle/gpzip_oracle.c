@@ -23,6 +23,8 @@
  *                          fails them (tests/test_oracle.py builds such a
  *                          mutant and checks that it does).
  *   orc_greedy_walk      - closed-form cases + the round trip.
+ *   orc_lookahead_walk   - (the ratio mode's rule, not the reference's)
+ *                          hand-derived cases + the round trip.
  *   orc_strict_walk      - the reference's exact-equality walk cases
  *                          (walk.rs:575-695) as fixtures.
  *   orc_build_code_lengths - reference property tests (huffman.rs:158-232),
@@ -208,6 +210,33 @@ size_t orc_greedy_walk(const uint32_t *pp, const uint8_t *in, size_t n, uint32_t
                 p += 1;
                 continue;
             }
+        }
+        out[k++] = t;
+        p += len;
+    }
+    return k;
+}
+
+/* The opt-in ratio mode's walk (this library's own rule, no reference
+ * counterpart): greedy_walk's demotion, extended to a match at p+2 that is
+ * longer than the one at p by two or more.  Positions past n count as
+ * literals.  Pinned by hand-derived cases and the round trip. */
+size_t orc_lookahead_walk(const uint32_t *pp, const uint8_t *in, size_t n, uint32_t *out) {
+    size_t p = 0, k = 0;
+    while (p < n) {
+        uint32_t t = pp[p];
+        uint32_t len = t >> 16;
+        if (len == 0) {
+            out[k++] = t;
+            p += 1;
+            continue;
+        }
+        uint32_t n1 = p + 1 < n ? pp[p + 1] >> 16 : 0;
+        uint32_t n2 = p + 2 < n ? pp[p + 2] >> 16 : 0;
+        if (n1 > len || n2 > len + 1) {
+            out[k++] = (uint32_t)in[p];
+            p += 1;
+            continue;
         }
         out[k++] = t;
         p += len;
@@ -692,15 +721,16 @@ size_t orc_chunk_bound(size_t n, size_t sub_chunk) {
     return 2 * n + members * (1024 + 18);
 }
 
-/* encode_one -- codec-gpu/src/gpu.rs:271-287 (host encoder path) */
-size_t orc_gzip_member(const uint8_t *in, size_t n, uint8_t *out, size_t out_cap) {
+/* encode_one -- codec-gpu/src/gpu.rs:271-287 (host encoder path); rule 2
+ * walks with orc_lookahead_walk (the ratio mode) instead of greedy_walk. */
+size_t orc_gzip_member_rule(const uint8_t *in, size_t n, int rule, uint8_t *out, size_t out_cap) {
     uint32_t *raw = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
     uint32_t *walked = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
     size_t dcap = member_bound(n);
     uint8_t *defl = (uint8_t *)malloc(dcap);
     size_t res = (size_t)-1;
     if (raw && walked && defl && orc_match_find(in, n, ORC_WINDOW, raw) == 0) {
-        size_t nw = orc_greedy_walk(raw, in, n, walked);
+        size_t nw = rule == 2 ? orc_lookahead_walk(raw, in, n, walked) : orc_greedy_walk(raw, in, n, walked);
         size_t dl = orc_encode_block_fast(walked, nw, defl, dcap);
         if (dl != (size_t)-1) res = orc_gzip_wrap(defl, dl, in, n, out, out_cap);
     }
@@ -710,16 +740,24 @@ size_t orc_gzip_member(const uint8_t *in, size_t n, uint8_t *out, size_t out_cap
     return res;
 }
 
+size_t orc_gzip_member(const uint8_t *in, size_t n, uint8_t *out, size_t out_cap) {
+    return orc_gzip_member_rule(in, n, 0, out, out_cap);
+}
+
 /* gzip_chunk_fn -- codec-gpu/src/gpu.rs:193-208 */
-size_t orc_gzip_chunk(const uint8_t *in, size_t n, size_t sub_chunk, uint8_t *out, size_t out_cap) {
+size_t orc_gzip_chunk_rule(const uint8_t *in, size_t n, size_t sub_chunk, int rule, uint8_t *out, size_t out_cap) {
     if (sub_chunk == 0) return (size_t)-1;
-    if (n <= sub_chunk) return orc_gzip_member(in, n, out, out_cap);
+    if (n <= sub_chunk) return orc_gzip_member_rule(in, n, rule, out, out_cap);
     size_t o = 0;
     for (size_t off = 0; off < n; off += sub_chunk) {
         size_t len = n - off < sub_chunk ? n - off : sub_chunk;
-        size_t m = orc_gzip_member(in + off, len, out + o, out_cap - o);
+        size_t m = orc_gzip_member_rule(in + off, len, rule, out + o, out_cap - o);
         if (m == (size_t)-1) return m;
         o += m;
     }
     return o;
+}
+
+size_t orc_gzip_chunk(const uint8_t *in, size_t n, size_t sub_chunk, uint8_t *out, size_t out_cap) {
+    return orc_gzip_chunk_rule(in, n, sub_chunk, 0, out, out_cap);
 }

@@ -50,6 +50,10 @@ size_t orc_greedy_walk(const uint32_t *per_pos, const uint8_t *in, size_t n, uin
 /* Strict walk, codec-gpu/src/gpu/walk.rs:512-525.  Returns token count. */
 size_t orc_strict_walk(const uint32_t *per_pos, size_t n, uint32_t *out);
 
+/* The opt-in ratio mode's walk (no reference counterpart): demote a match
+ * also when the match two positions ahead is longer by two or more. */
+size_t orc_lookahead_walk(const uint32_t *per_pos, const uint8_t *in, size_t n, uint32_t *out);
+
 /* reconstruct, codec-gpu/src/gpu/lz77.rs:288-304.  Returns bytes written or
  * (size_t)-1 when a distance reaches before the start / out_cap overflows. */
 size_t orc_reconstruct(const uint32_t *tok, size_t ntok, uint8_t *out, size_t out_cap);
@@ -76,6 +80,10 @@ size_t orc_gzip_wrap(const uint8_t *deflate, size_t dlen, const uint8_t *orig, s
 /* encode_one (gpu.rs:271-287) on one piece: match_find -> greedy_walk ->
  * encode_block_fast -> gzip_wrap.  Returns member bytes or (size_t)-1. */
 size_t orc_gzip_member(const uint8_t *in, size_t n, uint8_t *out, size_t out_cap);
+/* rule 0: the reference's lazy walk; rule 2: orc_lookahead_walk (ratio mode) */
+size_t orc_gzip_member_rule(const uint8_t *in, size_t n, int rule, uint8_t *out, size_t out_cap);
+size_t orc_gzip_chunk_rule(const uint8_t *in, size_t n, size_t sub_chunk, int rule, uint8_t *out,
+                           size_t out_cap);
 
 /* gzip_chunk_fn (gpu.rs:193-208): split into sub_chunk pieces, one member
  * each, concatenated.  Returns total bytes or (size_t)-1. */

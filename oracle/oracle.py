@@ -40,6 +40,10 @@ def _lib() -> C.CDLL:
         L.orc_greedy_walk.argtypes = [vp, vp, sz, vp]
         L.orc_strict_walk.restype = sz
         L.orc_strict_walk.argtypes = [vp, sz, vp]
+        L.orc_lookahead_walk.restype = sz
+        L.orc_lookahead_walk.argtypes = [vp, vp, sz, vp]
+        L.orc_gzip_chunk_rule.restype = sz
+        L.orc_gzip_chunk_rule.argtypes = [vp, sz, sz, C.c_int, vp, sz]
         L.orc_reconstruct.restype = sz
         L.orc_reconstruct.argtypes = [vp, sz, vp, sz]
         L.orc_build_code_lengths.argtypes = [vp, sz, C.c_uint32, vp]
@@ -97,6 +101,15 @@ def greedy_walk(tokens, data) -> np.ndarray:
     return out[:n].copy()
 
 
+def lookahead_walk(tokens, data) -> np.ndarray:
+    """The ratio mode's walk (this library's opt-in rule, not the reference's)."""
+    t = np.ascontiguousarray(tokens, np.uint32)
+    a = _u8(data)
+    out = np.zeros(max(t.size, 1), np.uint32)
+    n = _lib().orc_lookahead_walk(_p(t), _p(a), t.size, _p(out))
+    return out[:n].copy()
+
+
 def strict_walk(tokens) -> np.ndarray:
     t = np.ascontiguousarray(tokens, np.uint32)
     out = np.zeros(max(t.size, 1), np.uint32)
@@ -149,11 +162,12 @@ def gzip_member(data) -> bytes:
     return out[:n].tobytes()
 
 
-def gzip_chunk(data, sub_chunk: int = 131072) -> bytes:
+def gzip_chunk(data, sub_chunk: int = 131072, rule: int = 0) -> bytes:
+    """rule 0: the reference's walk; rule 2: the ratio mode's lookahead walk."""
     a = _u8(data)
     cap = _lib().orc_chunk_bound(a.size, sub_chunk)
     out = np.zeros(cap, np.uint8)
-    n = _lib().orc_gzip_chunk(_p(a), a.size, sub_chunk, _p(out), cap)
+    n = _lib().orc_gzip_chunk_rule(_p(a), a.size, sub_chunk, rule, _p(out), cap)
     assert n != SIZE_MAX
     return out[:n].tobytes()
 

@@ -7,6 +7,10 @@
 //                              stream to stdout, "gpu_chunks N cpu_chunks M
 //                              seconds S" to stderr
 //   test_backend decode FILE -- GzipDecompressor over FILE, output to stdout
+//   test_backend tar OUT INPUT...    -- TarBuilder stream (uncompressed) to OUT
+//   test_backend targz OUT CHUNK BATCH INPUT...
+//                           -- pack_tar_gz on the batched GPU form; prints
+//                              "batches B chunks C" to stderr
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -19,6 +23,7 @@
 #include <vector>
 
 #include "gpzip_backend.hpp"
+#include "gpzip_hip.h"
 
 using namespace gpzip;
 
@@ -285,6 +290,51 @@ int main(int argc, char **argv) {
         fwrite(buf->data(), 1, buf->size(), stdout);
         fprintf(stderr, "gpu_chunks %llu cpu_chunks %llu seconds %.6f\n", (unsigned long long)stats->gpu_chunks.load(),
                 (unsigned long long)stats->cpu_chunks.load(), sec);
+        return 0;
+    }
+    if (argc >= 4 && !strcmp(argv[1], "tar")) {
+        FILE *f = fopen(argv[2], "wb");
+        CHECK(f);
+        {
+            TarBuilder tb(std::make_unique<FileSink>(f));
+            for (int i = 3; i < argc; i++) {
+                std::string in = argv[i];
+                while (in.size() > 1 && in.back() == '/') in.pop_back();
+                const size_t k = in.rfind('/');
+                tb.append_path(in, k == std::string::npos ? in : in.substr(k + 1));
+            }
+            tb.finish();
+        }
+        fclose(f);
+        return 0;
+    }
+    if (argc >= 6 && !strcmp(argv[1], "targz")) {
+        gpz_config cfg;
+        gpz_config_default(&cfg);
+        gpz_ctx *ctx = nullptr;
+        CHECK(gpz_init(&cfg, &ctx) == GPZ_OK);
+        std::vector<std::string> inputs(argv + 5, argv + argc);
+        {
+            // the same composition as pack_tar_gz, keeping the writer's counters
+            FILE *f = fopen(argv[2], "wb");
+            CHECK(f);
+            auto gz = std::make_unique<BatchedChunkedWriter>(std::make_unique<FileSink>(f), ctx,
+                                                             (size_t)atol(argv[3]), (size_t)atol(argv[4]));
+            BatchedChunkedWriter *gzp = gz.get();
+            TarBuilder tb(std::move(gz));
+            for (const std::string &in0 : inputs) {
+                std::string in = in0;
+                while (in.size() > 1 && in.back() == '/') in.pop_back();
+                const size_t k = in.rfind('/');
+                tb.append_path(in, k == std::string::npos ? in : in.substr(k + 1));
+            }
+            tb.finish();
+            gzp->finalize();
+            fprintf(stderr, "batches %zu chunks %zu\n", gzp->batches_submitted(), gzp->chunks_submitted());
+            fclose(f);
+        }
+        pack_tar_gz(std::string(argv[2]) + ".2", inputs, ctx, (size_t)atol(argv[3]), (size_t)atol(argv[4]));
+        gpz_destroy(ctx);
         return 0;
     }
     if (argc >= 3 && !strcmp(argv[1], "decode")) {

@@ -625,3 +625,46 @@ def test_gunzip_decodes_every_form(ctx):
         assert gpzip_amd.gunzip(got[i]) == c.tobytes()
     whole = b"".join(got[i] for i in range(len(chunks)))
     assert gpzip_amd.gunzip(whole) == b"".join(c.tobytes() for c in chunks)
+
+
+# ---- the opt-in ratio mode (GPZ_CFG_RATIO; outside reference parity) ----------
+
+@pytest.mark.parametrize("kind", ["text", "binmix", "log"])
+@pytest.mark.parametrize("n", [4095, 4096, 4097, 8194, 70000, 131072])
+def test_lookahead_walk_equals_oracle(ctx, kind, n):
+    """The ratio mode's walk rule (strict = 2) on the GPU equals the oracle's
+    statement of it, across segment boundaries (the rule reads two tokens
+    ahead, so the summary runs after every token exists)."""
+    data = gpzip_amd.synth(kind, n, 31 + n)
+    t = O.match_find(data)
+    say("lookahead walk", kind, n)
+    assert_tokens_equal(ctx.walk(t, data, rule=2), O.lookahead_walk(t, data), f"{kind} {n}")
+
+
+def test_ratio_mode_forms_equal_oracle_and_are_smaller():
+    """A GPZ_CFG_RATIO context: plain, batched and no-alloc outputs equal the
+    oracle with the lookahead walk, decode, and are no larger than the
+    reference-parity output."""
+    import ctypes as Cc
+
+    c = gpzip_amd.Context(flags=gpzip_amd.GPZ_CFG_RATIO)
+    base = gpzip_amd.Context()
+    try:
+        chunks = [gpzip_amd.synth(k, s, 60 + i) for i, (k, s) in
+                  enumerate([("text", 2 << 20), ("binmix", 2 << 20), ("log", (2 << 20) - 9), ("text", 300001)])]
+        say("ratio mode")
+        ref = [O.gzip_chunk(x, 131072, rule=2) for x in chunks]
+        for x, r in zip(chunks, ref):
+            got = c.compress_chunk(x)
+            assert got == r
+            assert gzip.decompress(got) == x.tobytes()
+            assert len(got) <= len(base.compress_chunk(x))
+            out = np.empty(c.bound(x.size), np.uint8)
+            nn = Cc.c_size_t()
+            gpzip_amd._check(c.L.gpz_compress_chunk_noalloc(c.h, Cc.c_void_p(x.ctypes.data), x.size,
+                                                            Cc.c_void_p(out.ctypes.data), out.size, Cc.byref(nn)))
+            assert out[:nn.value].tobytes() == r
+        assert c.batch(chunks) == ref
+    finally:
+        c.close()
+        base.close()

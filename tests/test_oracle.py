@@ -385,3 +385,39 @@ def test_halving_mutant_fails_halving_kats(tmp_path):
 
     assert lens(cases["tie_leaf_first"]) == cases["tie_leaf_first"]["expect"]
     assert lens(cases["halve_once_limit7"]) != cases["halve_once_limit7"]["expect"]
+
+
+# the ratio mode's walk (this library's opt-in rule, not the reference's)
+LOOKAHEAD_CASES = [
+    # p0 (3,1): the next position is a literal, but p2 holds (5,2), longer by 2:
+    # p0 is demoted; p1 a literal; p2 (5,2) with literals after it is taken
+    ("demote_for_two_ahead", b"abcdefghij",
+     [[3, 1], [0, 0x62], [5, 2], [0, 0x64], [0, 0x65], [0, 0x66], [0, 0x67], [0, 0x68], [0, 0x69], [0, 0x6A]],
+     [[0, 0x61], [0, 0x62], [5, 2], [0, 0x68], [0, 0x69], [0, 0x6A]]),
+    # a match two ahead longer by only 1 does not demote
+    ("longer_by_one_keeps", b"abcdefgh",
+     [[3, 1], [0, 0x62], [4, 2], [0, 0x64], [0, 0x65], [0, 0x66], [0, 0x67], [0, 0x68]],
+     [[3, 1], [0, 0x64], [0, 0x65], [0, 0x66], [0, 0x67], [0, 0x68]]),
+    # the reference's rule still applies one ahead; near the end the positions
+    # past the input count as literals
+    ("end_of_input", b"wxyz",
+     [[3, 1], [0, 0x78], [0, 0x79], [0, 0x7A]],
+     [[3, 1], [0, 0x7A]]),
+]
+
+
+@pytest.mark.parametrize("name,data,toks,exp", LOOKAHEAD_CASES, ids=[c[0] for c in LOOKAHEAD_CASES])
+def test_lookahead_walk_cases(name, data, toks, exp):
+    assert (O.lookahead_walk(pack(toks), data) == pack(exp)).all()
+
+
+@pytest.mark.parametrize("kind", ["text", "binmix", "log"])
+def test_lookahead_walk_round_trip_and_smaller(kind):
+    import gpzip_amd
+
+    data = gpzip_amd.synth(kind, 1 << 18, 8)
+    t = O.match_find(data)
+    w2 = O.lookahead_walk(t, data)
+    assert O.reconstruct(w2, data.size) == data.tobytes()
+    g0, g2 = O.gzip_chunk(data), O.gzip_chunk(data, rule=2)
+    assert gzip.decompress(g2) == data.tobytes() and len(g2) <= len(g0)
