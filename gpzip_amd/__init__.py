@@ -52,11 +52,11 @@ class Desc(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("k0_grid", C.c_uint32), ("k1_grid", C.c_uint32), ("k3_grid", C.c_uint32), ("emit_grid", C.c_uint32),
-                ("k1_variant", C.c_uint32), ("emit_variant", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("k1_variant", C.c_uint32), ("emit_variant", C.c_uint32), ("k0_variant", C.c_uint32),
+                ("k1_seg_per_wg", C.c_uint32), ("k3_seg_per_wg", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
     def as_dict(self) -> dict:
-        return {"k0_grid": self.k0_grid, "k1_grid": self.k1_grid, "k3_grid": self.k3_grid,
-                "emit_grid": self.emit_grid, "k1_variant": self.k1_variant, "emit_variant": self.emit_variant}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class ServerConfig(C.Structure):

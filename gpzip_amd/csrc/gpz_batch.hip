@@ -288,9 +288,9 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
         p->grids.emit = p->grids.huff = p->grids.chain = std::max<uint32_t>(1, p->pl.M);
     }
     if (tune) {
-        if (tune->k1_variant > 2 || tune->emit_variant > 2) {
+        if (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3) {
             delete p;
-            return fail(GPZ_E_INVAL, "k1_variant and emit_variant must be 0, 1 or 2");
+            return fail(GPZ_E_INVAL, "k1_variant and emit_variant must be 0-2, k0_variant 0-3");
         }
         apply_tune(p->grids, *tune, p->pl.G, p->pl.M);
     } else if (flags & GPZ_PLAN_PERSISTENT) {
@@ -343,48 +343,60 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
     struct Cand {
         uint32_t grid, variant;
     };
-    {  // match-find: occupancy variant x persistent grid
-        const Cand cs[] = {{cap((uint64_t)cu * 2, G), 0}, {cap((uint64_t)cu * 4, G), 0}, {cap((uint64_t)cu * 8, G), 0},
-                           {G, 0},  {cap(cu, G), 1}, {cap((uint64_t)cu * 2, G), 1}, {cap((uint64_t)cu * 8, G), 1}};
+    // segments per workgroup -> grid on this batch
+    auto spw = [&](uint32_t n) { return (G + n - 1) / n; };
+    uint32_t best_k1_spw = 0, best_k3_spw = 0;
+    {  // match-find: occupancy variant x segments per workgroup (1 = one workgroup per segment)
+        const uint32_t spws[] = {1, 2, 4, 8, 16, 32, 64};
         float bt = 1e30f;
-        for (const Cand &c : cs) {
-            Grids g = best;
-            g.k1 = c.grid;
-            g.k1v = c.variant;
-            float t;
-            if ((rc = timed(g, 5, 1, &t))) break;
-            if (t < bt) {
-                bt = t;
-                best.k1 = g.k1;
-                best.k1v = g.k1v;
+        for (uint32_t v = 0; v < 2 && !rc; v++) {
+            for (uint32_t n : spws) {
+                if (n > 1 && spw(n) < cu / 2) continue;  // too few workgroups to fill the chip
+                Grids g = best;
+                g.k1 = spw(n);
+                g.k1v = v;
+                float t;
+                if ((rc = timed(g, 5, 1, &t))) break;
+                if (t < bt) {
+                    bt = t;
+                    best.k1 = g.k1;
+                    best.k1v = g.k1v;
+                    best_k1_spw = n;
+                }
             }
         }
     }
-    if (!rc) {  // segment tables
-        const uint32_t cs[] = {cap((uint64_t)cu * 3, G), cap((uint64_t)cu * 6, G), cap((uint64_t)cu * 12, G), G};
+    if (!rc) {  // segment tables: workgroup size x persistent grid
+        const Cand cs[] = {{cap((uint64_t)cu * 6, G), 0}, {cap((uint64_t)cu * 12, G), 0}, {G, 0},
+                           {cap((uint64_t)cu * 12, G), 1}, {cap((uint64_t)cu * 24, G), 1}, {G, 1},
+                           {cap((uint64_t)cu * 3, G), 2}, {cap((uint64_t)cu * 6, G), 2}, {G, 2}};
         float bt = 1e30f;
-        for (uint32_t c : cs) {
+        for (const Cand &c : cs) {
             Grids g = best;
-            g.k0 = c;
+            g.k0 = c.grid;
+            g.k0v = c.variant;
             float t;
             if ((rc = timed(g, 0, 5, &t))) break;
             if (t < bt) {
                 bt = t;
-                best.k0 = c;
+                best.k0 = g.k0;
+                best.k0v = g.k0v;
             }
         }
     }
-    if (!rc) {  // walk / token emission (with chain, Huffman and scan in the interval)
-        const uint32_t cs[] = {cap((uint64_t)cu * 2, G), cap((uint64_t)cu * 4, G), cap((uint64_t)cu * 8, G), G};
+    if (!rc) {  // walk / token emission (with chain, Huffman and scan in the interval): segments per workgroup
+        const uint32_t spws[] = {1, 2, 4, 8, 16, 32};
         float bt = 1e30f;
-        for (uint32_t c : cs) {
+        for (uint32_t n : spws) {
+            if (n > 1 && spw(n) < cu / 2) continue;
             Grids g = best;
-            g.k3 = c;
+            g.k3 = spw(n);
             float t;
             if ((rc = timed(g, 1, 3, &t))) break;
             if (t < bt) {
                 bt = t;
-                best.k3 = c;
+                best.k3 = g.k3;
+                best_k3_spw = n;
             }
         }
     }
@@ -416,6 +428,9 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
     out->emit_grid = best.emit;
     out->k1_variant = best.k1v + 1;
     out->emit_variant = best.emitv + 1;
+    out->k0_variant = best.k0v == 0 ? 2 : (best.k0v == 1 ? 1 : 3);
+    out->k1_seg_per_wg = best_k1_spw;
+    out->k3_seg_per_wg = best_k3_spw;
     if (ms) {
         ms[0] = t1;
         ms[1] = t0;

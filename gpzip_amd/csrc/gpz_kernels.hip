@@ -38,10 +38,8 @@ namespace gpz {
 // constants
 
 __constant__ uint32_t c_crc_tab[256];   // reflected CRC-32 byte table
-__constant__ uint32_t c_crc_fold[512];  // x^(8*8*(511-t)) mod P, reflected
-__constant__ uint32_t c_xpow[32];       // x^(8*2^k) mod P, reflected
+__constant__ uint32_t c_xpowb[4096];    // x^(8*b) mod P, reflected, b < 4096 (bytes within a segment)
 __constant__ uint32_t c_segpow[4097];   // x^(8*4096*j) mod P, reflected (segments of 4 KiB)
-__constant__ uint32_t c_bytepow[8];     // x^(8*b) mod P, reflected, b < 8
 
 __constant__ uint16_t c_len_base[29] = {3,  4,  5,  6,  7,  8,  9,  10, 11,  13,  15,  17,  19,  23, 27,
                                         31, 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
@@ -82,13 +80,6 @@ __device__ __forceinline__ uint32_t gf2_mul(uint32_t a, uint32_t b) {
         r ^= b & (0u - ((a >> (31 - i)) & 1u));
         b = (b >> 1) ^ (0xEDB88320u & (0u - (b & 1u)));
     }
-    return r;
-}
-// x^(8n) mod P
-__device__ __forceinline__ uint32_t xpow8(uint64_t n) {
-    uint32_t r = 0x80000000u;
-    for (int k = 0; n; k++, n >>= 1)
-        if (n & 1) r = gf2_mul(r, c_xpow[k]);
     return r;
 }
 
@@ -345,7 +336,6 @@ __device__ __forceinline__ uint32_t block_walk_count(uint32_t J, uint32_t bs, ui
 // segment), built once per segment and streamed by the 9 segments whose
 // window covers it.  Also computes the segment's raw CRC-32.
 
-constexpr int K0_T = 512;
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
 // hash3 of lz77_hash_build.wgsl:36-43 on three bytes
@@ -357,9 +347,14 @@ __device__ __forceinline__ uint32_t hash3(uint32_t b0, uint32_t b1, uint32_t b2)
 // oldest/newest p+1 per (hash, segment) by global atomics) restated per
 // segment in LDS: the set of 16-bit hashes of the segment as a bitmap, their
 // ranks, and per hash (in hash order) its first and last position.  Also the
-// raw CRC-32 of the segment's bytes.
-__global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
-    __shared__ uint32_t bm[2048];
+// raw CRC-32 of the segment's bytes.  BT threads (256, 512 or 1024; the
+// autotuner picks), PPT = 4096 / BT positions each.
+template <int BT>
+__global__ __launch_bounds__(BT) void k_seg_table(Work w) {
+    constexpr uint32_t PPT = SEG / BT;           // positions per thread
+    constexpr uint32_t NW = (PPT + 2 + 3) / 4;   // words holding them and the two bytes after
+    constexpr uint32_t WPT = 2048 / BT;          // bitmap words per thread
+    __shared__ __align__(16) uint32_t bm[2048];
     __shared__ uint16_t rk[2048];
     __shared__ uint32_t fst[SEG];
     __shared__ uint32_t lst[SEG];
@@ -367,96 +362,91 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
     __shared__ uint32_t tmp[16];
     const uint32_t tid = threadIdx.x;
     const uint32_t G = uni(w.counts->nseg);
-    for (uint32_t i = tid; i < 256; i += K0_T) crc_tab[i] = c_crc_tab[i];
-    // bytes x0 .. x0+11 of a segment (buffers are padded past every member),
-    // loaded one segment ahead so the loads are in flight while the current
-    // one is processed
-    const uint32_t x0 = tid * 8;
-    auto fetch = [&](uint32_t gg, uint32_t &a, uint32_t &b, uint32_t &c) {
-        a = b = c = 0;
+    for (uint32_t i = tid; i < 256; i += BT) crc_tab[i] = c_crc_tab[i];
+    // bytes x0 .. x0 + 4*NW - 1 of a segment (buffers are padded past every
+    // member), loaded one segment ahead so the loads are in flight while the
+    // current one is processed
+    const uint32_t x0 = tid * PPT;
+    auto fetch = [&](uint32_t gg, uint32_t (&d)[NW]) {
+#pragma unroll
+        for (uint32_t j = 0; j < NW; j++) d[j] = 0;
         if (gg >= G) return;
         const MemberDesc mm = load_member(w.mem, sload_u32(w.seg_mem, gg));
         const uint32_t SS = (gg - mm.seg_base) << SEG_LOG2, LL = min(SS + SEG, mm.n) - SS;
         if (x0 < LL) {
             const uint32_t *src = reinterpret_cast<const uint32_t *>(w.in + mm.in_off + SS + x0);
-            a = src[0];
-            b = src[1];
-            c = src[2];
+#pragma unroll
+            for (uint32_t j = 0; j < NW; j++) d[j] = src[j];
         }
     };
-    const uint32_t fold = c_crc_fold[tid];  // x^(8 * bytes after this thread's 8) for a full segment
-    uint32_t n0, n1, n2;
-    fetch(xcd_block(), n0, n1, n2);
+    uint32_t nx[NW];
+    fetch(xcd_block(), nx);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
         const MemberDesc md = load_member(w.mem, sload_u32(w.seg_mem, g));
         const uint32_t n = md.n, s = g - md.seg_base;
         const uint32_t S = s << SEG_LOG2, E = min(S + SEG, n), L = E - S;
         uint32_t *T = w.segtab + (uint64_t)g * TAB_WORDS;
-        const uint32_t d0 = n0, d1 = n1, d2 = n2;
-        const uint64_t lo = (uint64_t)d0 | ((uint64_t)d1 << 32);
+        uint32_t d[NW];
 #pragma unroll
-        for (int r = 0; r < 4; r++) bm[tid + r * K0_T] = 0;
+        for (uint32_t j = 0; j < NW; j++) d[j] = nx[j];
+        auto byte_at = [&](uint32_t i) { return (d[i >> 2] >> (8 * (i & 3))) & 0xFFu; };
+#pragma unroll
+        for (uint32_t r = 0; r < WPT; r++) bm[tid + r * BT] = 0;
         __syncthreads();
         // raw CRC-32 of the segment, started here so that its chain of table
-        // lookups overlaps the atomics and barriers below: 8 bytes per thread,
-        // folded by x^(8 * bytes after them), reduced at the end
+        // lookups overlaps the atomics and barriers below: PPT bytes per
+        // thread, folded by x^(8 * bytes after them), reduced at the end
         uint32_t crc = 0;
         {
-            const uint64_t hi = d2;
 #pragma unroll
-            for (int b = 0; b < 8; b++) {
-                if (x0 + b < L) {
-                    uint32_t byte = (uint32_t)((b < 8 ? lo : hi) >> (8 * b)) & 0xFF;
-                    crc = crc_tab[(crc ^ byte) & 0xFF] ^ (crc >> 8);
-                }
+            for (uint32_t b = 0; b < PPT; b++) {
+                if (x0 + b < L) crc = crc_tab[(crc ^ byte_at(b)) & 0xFF] ^ (crc >> 8);
             }
-            if (x0 < L) {
-                const uint32_t after = L - min(x0 + 8, L);
-                crc = gf2_mul(crc, L == SEG ? fold : xpow8(after));
-            } else {
-                crc = 0;
-            }
+            crc = x0 < L ? gf2_mul(crc, c_xpowb[L - min(x0 + PPT, L)]) : 0;
         }
-        uint32_t hh[8];
+        uint32_t hh[PPT];
         uint32_t hvm = 0;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (uint32_t i = 0; i < PPT; i++) {
             const uint32_t x = x0 + i;
             const bool hv = x < L && S + x + 2 < n;
-            const uint32_t b0 = (uint32_t)(lo >> (8 * i)) & 0xFF;
-            const uint32_t b1 = i + 1 < 8 ? (uint32_t)(lo >> (8 * (i + 1))) & 0xFF : d2 & 0xFF;
-            const uint32_t b2 = i + 2 < 8 ? (uint32_t)(lo >> (8 * (i + 2))) & 0xFF : (d2 >> (8 * (i + 2 - 8))) & 0xFF;
-            hh[i] = hash3(b0, b1, b2);
+            hh[i] = hash3(byte_at(i), byte_at(i + 1), byte_at(i + 2));
             hvm |= (uint32_t)hv << i;
             if (hv) atomicOr(&bm[hh[i] >> 5], 1u << (hh[i] & 31));
         }
         // the bytes are in registers now: the next segment's loads go out
         // without a wait for them
         asm volatile("" ::: "memory");
-        fetch(g + gridDim.x, n0, n1, n2);
+        fetch(g + gridDim.x, nx);
         __syncthreads();
         uint32_t D;  // distinct hashes in the segment
         {
-            // ranks of words 4t .. 4t+3; bitmap and ranks go out as they are
-            const uint4 q = reinterpret_cast<const uint4 *>(bm)[tid];
-            const uint32_t c0 = __popc(q.x), c1 = __popc(q.y), c2 = __popc(q.z);
-            const uint32_t ex = block_scan_dpp<K0_T>(c0 + c1 + c2 + __popc(q.w), tmp, &D);
-            const uint32_t r0 = ex, r1 = ex + c0, r2 = r1 + c1, r3 = r2 + c2;
-            rk[4 * tid] = (uint16_t)r0;
-            rk[4 * tid + 1] = (uint16_t)r1;
-            rk[4 * tid + 2] = (uint16_t)r2;
-            rk[4 * tid + 3] = (uint16_t)r3;
-            reinterpret_cast<uint4 *>(T + TAB_BITS)[tid] = q;
-            reinterpret_cast<uint2 *>(T + TAB_RANK)[tid] = make_uint2(r0 | (r1 << 16), r2 | (r3 << 16));
+            // ranks of words WPT*t .. WPT*t + WPT - 1; bitmap and ranks go out as they are
+            uint32_t q[WPT], c[WPT], tot = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < WPT; j++) {
+                q[j] = bm[WPT * tid + j];
+                c[j] = __popc(q[j]);
+                tot += c[j];
+            }
+            const uint32_t ex = block_scan_dpp<BT>(tot, tmp, &D);
+            uint32_t r = ex;
+#pragma unroll
+            for (uint32_t j = 0; j < WPT; j++) {
+                rk[WPT * tid + j] = (uint16_t)r;
+                T[TAB_BITS + WPT * tid + j] = q[j];
+                if (j & 1) T[TAB_RANK + (WPT * tid + j) / 2] = (r << 16) | (r - c[j - 1]);
+                r += c[j];
+            }
             if (tid == 0) w.seg_ntab[g] = D;
-            for (uint32_t id = tid; id < D; id += K0_T) {
+            for (uint32_t id = tid; id < D; id += BT) {
                 fst[id] = EMPTY;
                 lst[id] = 0;
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (uint32_t i = 0; i < PPT; i++) {
             if (!((hvm >> i) & 1u)) continue;
             const uint32_t h = hh[i];
             const uint32_t id = rk[h >> 5] + __popc(bm[h >> 5] & ((1u << (h & 31)) - 1u));
@@ -464,16 +454,16 @@ __global__ __launch_bounds__(K0_T) void k_seg_table(Work w) {
             atomicMax(&lst[id], x0 + i);
         }
         __syncthreads();
-        for (uint32_t id = tid; id < D; id += K0_T) T[TAB_ENT + id] = (fst[id] << 12) | lst[id];
+        for (uint32_t id = tid; id < D; id += BT) T[TAB_ENT + id] = (fst[id] << 12) | lst[id];
         {
             uint32_t c = crc;
 #pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) c ^= __shfl_xor(c, d, 64);
+            for (int dd = 32; dd >= 1; dd >>= 1) c ^= __shfl_xor(c, dd, 64);
             if ((tid & 63) == 0) tmp[tid >> 6] = c;
             __syncthreads();
             if (tid == 0) {
                 uint32_t t = 0;
-                for (int i = 0; i < K0_T / 64; i++) t ^= tmp[i];
+                for (int i = 0; i < BT / 64; i++) t ^= tmp[i];
                 w.seg_crc[g] = t;
             }
         }
@@ -1586,10 +1576,7 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
         uint32_t crc = 0;
         if (w.gzip && md.n) {
             const uint32_t Lb = md.n - ((md.nseg - 1) << SEG_LOG2);  // 1 .. 4096
-            const uint32_t q = Lb >> 3;
-            const uint32_t XL = q ? gf2_mul(q < 512 ? c_crc_fold[511 - q] : c_segpow[1] /* Lb = 4096 */,
-                                            c_bytepow[Lb & 7u])
-                                  : c_bytepow[Lb & 7u];
+            const uint32_t XL = Lb < SEG ? c_xpowb[Lb] : c_segpow[1];  // x^(8 Lb)
             for (uint32_t s = lane; s < md.nseg; s += 64) {
                 const uint32_t raw = w.seg_crc[md.seg_base + s];
                 crc ^= s + 1 == md.nseg ? raw : gf2_mul(raw, gf2_mul(c_segpow[md.nseg - 2 - s], XL));
@@ -1845,7 +1832,7 @@ __global__ __launch_bounds__(256) void k_plan_one(Work w, uint32_t len, uint32_t
 namespace gpz {
 
 hipError_t upload_constants() {
-    uint32_t tab[256], fold[512], xp[32];
+    uint32_t tab[256];
     for (uint32_t i = 0; i < 256; i++) {
         uint32_t c = i;
         for (int k = 0; k < 8; k++) c = (c & 1u) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
@@ -1859,31 +1846,27 @@ hipError_t upload_constants() {
         }
         return r;
     };
-    xp[0] = 0x00800000u;  // x^8
-    for (int k = 1; k < 32; k++) xp[k] = mul(xp[k - 1], xp[k - 1]);
-    auto pw = [&](uint64_t n) {
-        uint32_t r = 0x80000000u;
-        for (int k = 0; n; k++, n >>= 1)
-            if (n & 1) r = mul(r, xp[k]);
-        return r;
-    };
-    for (uint32_t t = 0; t < 512; t++) fold[t] = pw(8ull * (511 - t));
-    static uint32_t segpow[4097];
-    uint32_t bytepow[8];
+    // x^(8b) for b < 4096 by repeated multiplication with x^8, then whole segments
+    static uint32_t xpowb[4096], segpow[4097];
+    const uint32_t x8 = 0x00800000u;  // x^8, reflected
+    xpowb[0] = 0x80000000u;           // 1
+    for (uint32_t b = 1; b < 4096; b++) xpowb[b] = mul(xpowb[b - 1], x8);
+    const uint32_t seg1 = mul(xpowb[4095], x8);  // x^(8 * 4096)
     segpow[0] = 0x80000000u;
-    const uint32_t seg1 = pw(4096);
     for (uint32_t j = 1; j <= 4096; j++) segpow[j] = mul(segpow[j - 1], seg1);
-    for (uint32_t b = 0; b < 8; b++) bytepow[b] = pw(b);
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_tab), tab, sizeof(tab));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_segpow), segpow, sizeof(segpow));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_bytepow), bytepow, sizeof(bytepow));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_fold), fold, sizeof(fold));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_xpow), xp, sizeof(xp));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_xpowb), xpowb, sizeof(xpowb));
     return e;
 }
 
-void launch_seg_table(const Work &w, uint32_t grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_seg_table, dim3(grid), dim3(K0_T), 0, st, w);
+void launch_seg_table(const Work &w, uint32_t grid, int variant, hipStream_t st) {
+    if (variant == 1)
+        hipLaunchKernelGGL(k_seg_table<256>, dim3(grid), dim3(256), 0, st, w);
+    else if (variant == 2)
+        hipLaunchKernelGGL(k_seg_table<1024>, dim3(grid), dim3(1024), 0, st, w);
+    else
+        hipLaunchKernelGGL(k_seg_table<512>, dim3(grid), dim3(512), 0, st, w);
 }
 void launch_match_find(const Work &w, uint32_t grid, int summary, int variant, hipStream_t st) {
     if (variant == 1)

@@ -50,9 +50,10 @@ struct Grids {
     uint32_t k0 = 1, k1 = 1, k3 = 1, chain = 1, huff = 1, emit = 1;
     uint32_t k1v = 0;    // match-find variant: 0 two workgroups per CU, 1 one
     uint32_t emitv = 0;  // bit-emission variant: 0 1024 threads per member, 1 256
+    uint32_t k0v = 0;    // segment-table workgroup: 0 512 threads, 1 256, 2 1024
     bool operator==(const Grids &o) const {
         return k0 == o.k0 && k1 == o.k1 && k3 == o.k3 && chain == o.chain && huff == o.huff && emit == o.emit &&
-               k1v == o.k1v && emitv == o.emitv;
+               k1v == o.k1v && emitv == o.emitv && k0v == o.k0v;
     }
 };
 

@@ -28,7 +28,7 @@
 namespace gpz {
 
 hipError_t upload_constants();
-void launch_seg_table(const Work &w, uint32_t grid, hipStream_t st);
+void launch_seg_table(const Work &w, uint32_t grid, int variant, hipStream_t st);
 void launch_match_find(const Work &w, uint32_t grid, int summary, int variant, hipStream_t st);
 void launch_walk_summary(const Work &w, uint32_t grid, hipStream_t st);
 void launch_chain(const Work &w, uint32_t grid, hipStream_t st);
@@ -293,8 +293,12 @@ void apply_tune(Grids &g, const gpz_tune &t, uint64_t G, uint64_t M) {
     if (t.k1_grid) g.k1 = t.k1_grid;
     if (t.k3_grid) g.k3 = t.k3_grid;
     if (t.emit_grid) g.emit = t.emit_grid;
+    // segments per workgroup: the grid follows the batch
+    if (t.k1_seg_per_wg) g.k1 = (uint32_t)((G + t.k1_seg_per_wg - 1) / t.k1_seg_per_wg);
+    if (t.k3_seg_per_wg) g.k3 = (uint32_t)((G + t.k3_seg_per_wg - 1) / t.k3_seg_per_wg);
     if (t.k1_variant == 1 || t.k1_variant == 2) g.k1v = t.k1_variant - 1;
     if (t.emit_variant == 1 || t.emit_variant == 2) g.emitv = t.emit_variant - 1;
+    if (t.k0_variant >= 1 && t.k0_variant <= 3) g.k0v = t.k0_variant == 2 ? 0 : (t.k0_variant == 1 ? 1 : 2);
     // a record tuned on a larger batch must not launch workgroups with no
     // segment or member to take
     auto cap = [](uint32_t v, uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(v, n)); };
@@ -310,7 +314,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
     if (ev) hipEventRecord(ev[0], st);
     switch (mode) {
         case MODE_MATCH:
-            launch_seg_table(w, g.k0, st);
+            launch_seg_table(w, g.k0, g.k0v, st);
             launch_match_find(w, g.k1, 0, g.k1v, st);
             break;
         case MODE_WALK:
@@ -327,7 +331,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
             if (ev) hipEventRecord(ev[4], st);
             break;
         default:  // MODE_FULL
-            launch_seg_table(w, g.k0, st);
+            launch_seg_table(w, g.k0, g.k0v, st);
             if (ev) hipEventRecord(ev[5], st);
             if (w.strict == 2) {
                 // ratio mode: the walk rule reads two tokens ahead, so the
@@ -508,8 +512,8 @@ extern "C" {
 int gpz_set_tune(gpz_ctx *ctx, const gpz_tune *tune) {
     clear_error();
     if (!ctx) return fail(GPZ_E_INVAL, "null context");
-    if (tune && (tune->k1_variant > 2 || tune->emit_variant > 2))
-        return fail(GPZ_E_INVAL, "k1_variant and emit_variant must be 0, 1 or 2");
+    if (tune && (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3))
+        return fail(GPZ_E_INVAL, "k1_variant and emit_variant must be 0-2, k0_variant 0-3");
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (tune) ctx->tune = *tune;
     else ctx->tune = gpz_tune{};

@@ -185,7 +185,9 @@ typedef struct gpz_plan gpz_plan;
  * 15 % of the pipeline, so plans that are not being profiled leave it off. */
 enum { GPZ_PLAN_GRAPH = 1u, GPZ_PLAN_PERSISTENT = 2u, GPZ_PLAN_TIMED = 4u };
 
-/* Launch shape of the batched form.  Zero fields keep the built-in choice. */
+/* Launch shape of the batched form.  Zero fields keep the built-in choice.
+ * Grids are absolute; segments-per-workgroup fields give the grid relative
+ * to each batch (ceil(segments / n)) and take precedence over the grid. */
 typedef struct gpz_tune {
     uint32_t k0_grid;    /* persistent grid of the segment-table kernel */
     uint32_t k1_grid;    /* persistent grid of the match-find kernel */
@@ -195,7 +197,11 @@ typedef struct gpz_tune {
                             2 = one workgroup per CU (128 VGPRs); 0 = built-in */
     uint32_t emit_variant; /* bit-emission workgroup: 1 = 1024 threads per member,
                               2 = 256 threads (eight members per CU); 0 = built-in (1) */
-    uint32_t reserved[2];
+    uint32_t k0_variant;   /* segment-table workgroup: 1 = 256 threads (16 positions each),
+                              2 = 512 (8), 3 = 1024 (4); 0 = built-in (2) */
+    uint32_t k1_seg_per_wg; /* match-find segments per workgroup; 0 = use k1_grid */
+    uint32_t k3_seg_per_wg; /* walk/emit-tokens segments per workgroup; 0 = use k3_grid */
+    uint32_t reserved[3];
 } gpz_tune;
 
 /* Tuning record used by every batched launch of this context (gpz_batch_*,

@@ -213,6 +213,7 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     probe = torch.zeros(4 * o + (1 << 20), dtype=torch.uint8, device="cuda:0")
     tune, t_ms, t_def = ctx.autotune(descs, d_in.data_ptr(), probe.data_ptr())
     assert tune.k1_variant in (1, 2) and tune.emit_variant in (1, 2) and tune.k1_grid >= 1 and tune.k0_grid >= 1
+    assert tune.k0_variant in (1, 2, 3) and tune.k1_seg_per_wg >= 1 and tune.k3_seg_per_wg >= 1
     assert tune.k3_grid >= 1 and tune.emit_grid >= 1 and t_ms > 0 and t_def > 0
     assert run(tune) == ref
     for v in (1, 2):
@@ -221,6 +222,14 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
             t.k0_grid, t.k1_grid, t.k3_grid, t.emit_grid, t.k1_variant = g, g, g, max(1, g // 3), v
             t.emit_variant = 3 - v
             say("shape", v, g)
+            assert run(t) == ref
+    # segment-table workgroup sizes and segments per workgroup
+    for k0v in (1, 2, 3):
+        for spw in (1, 3, 64):
+            t = gpzip_amd.Tune()
+            t.k0_variant, t.k0_grid = k0v, 97 * k0v
+            t.k1_seg_per_wg, t.k3_seg_per_wg = spw, 4 - k0v
+            say("shape k0v", k0v, "spw", spw)
             assert run(t) == ref
     # the context-wide record drives the batch API (graph re-captured on change)
     t = gpzip_amd.Tune()
@@ -232,9 +241,9 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     finally:
         ctx.set_tune(None)
     assert ctx.get_tune().k1_grid == 0
-    for field in ("k1_variant", "emit_variant"):
+    for field, v in (("k1_variant", 3), ("emit_variant", 3), ("k0_variant", 4)):
         bad = gpzip_amd.Tune()
-        setattr(bad, field, 3)
+        setattr(bad, field, v)
         with pytest.raises(gpzip_amd.GpzError):
             ctx.set_tune(bad)
 
