@@ -444,10 +444,23 @@ def hybrid_sweep(mib: int = 256, settings=((0, 0, 4), (1, 0, 4), (8, 0, 4), (16,
             assert gpzip_amd.gunzip(r.stdout) == data.tobytes(), "hybrid stream does not decode"
             g, c = int(st["gpu_chunks"]), int(st["cpu_chunks"])
             res.append({"gpu_permits": p, "gpu_wait_ms": wait, "warmup_chunks": warmup, "gbps": round(data.size / float(st["seconds"]) / 1e9, 3),
+                        "cold_gbps": round(data.size / float(st["cold_seconds"]) / 1e9, 3),
                         "gpu_chunk_share": round(g / max(1, g + c), 3), "ratio": round(len(r.stdout) / data.size, 5)})
+        # `--backend gpu`: GpuBackend's own writer (128 KiB members, 16 in flight, gpu.rs:160-166)
+        log("backend gpu ...")
+        r = subprocess.run([exe, "gpu", f], capture_output=True, timeout=600)
+        if r.returncode:
+            raise RuntimeError(r.stderr.decode())
+        kv = r.stderr.decode().split()
+        st = dict(zip(kv[0::2], kv[1::2]))
+        assert gpzip_amd.gunzip(r.stdout) == data.tobytes(), "backend gpu stream does not decode"
+        gpu_only = {"gbps": round(data.size / float(st["seconds"]) / 1e9, 3),
+                    "cold_gbps": round(data.size / float(st["cold_seconds"]) / 1e9, 3), "ratio": round(len(r.stdout) / data.size, 5),
+                    "writer": "GpuBackend compressor: ParallelChunkedWriter of 2 MiB chunks (16 members of 128 KiB, "
+                              "the bytes of 128 KiB chunks), 16 in flight on 16 lanes"}
     cores, _ = host_cores()
     return {"workload": f"`--backend auto`: binmix {mib} MiB from host memory, 2 MiB chunks, {cores} CPU workers "
-                        f"(zlib level 5) + GPU permits", "runs": res}
+                        f"(zlib level 5) + GPU permits", "runs": res, "backend_gpu": gpu_only}
 
 
 def mixed_sizes(torch, ctx, steps):

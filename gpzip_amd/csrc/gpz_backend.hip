@@ -73,8 +73,12 @@ void ParallelChunkedWriter::worker() {
         } catch (...) {
             err = std::current_exception();
         }
-        job.second.reset();
         lk.lock();
+        if (spare_.size() <= max_in_flight_) {
+            job.second->clear();
+            spare_.push_back(std::move(*job.second));
+        }
+        job.second.reset();
         if (err) {
             if (!sticky_) sticky_ = err;
         } else {
@@ -90,9 +94,14 @@ void ParallelChunkedWriter::dispatch_current() {
     if (current_.empty()) return;
     uint32_t idx = next_index_++;
     auto bytes = std::make_shared<std::vector<uint8_t>>(std::move(current_));
-    current_ = std::vector<uint8_t>();
-    current_.reserve(chunk_size_);
     std::unique_lock<std::mutex> lk(mu_);
+    if (!spare_.empty()) {
+        current_ = std::move(spare_.back());
+        spare_.pop_back();
+    } else {
+        current_ = std::vector<uint8_t>();
+        current_.reserve(chunk_size_);
+    }
     in_flight_++;
     jobs_.emplace_back(idx, std::move(bytes));
     if (workers_.size() < max_in_flight_ && workers_.size() < in_flight_) workers_.emplace_back([this] { worker(); });
@@ -177,7 +186,7 @@ class GpuGzipCompressor : public Compressor {
 std::shared_ptr<GpuBackend> GpuBackend::try_init(uint32_t lanes) {
     gpz_config cfg;
     gpz_config_default(&cfg);
-    if (lanes) cfg.lanes = lanes;
+    cfg.lanes = lanes ? lanes : cfg.max_in_flight;
     gpz_ctx *ctx = nullptr;
     int rc = gpz_init(&cfg, &ctx);
     if (rc != GPZ_OK) throw Error(Error::Kind::Codec, std::string("gpu init failed: ") + gpz_last_error());
@@ -185,6 +194,7 @@ std::shared_ptr<GpuBackend> GpuBackend::try_init(uint32_t lanes) {
     b->ctx_ = ctx;
     b->chunk_size_ = cfg.sub_chunk;
     b->max_in_flight_ = cfg.max_in_flight;
+    b->writer_chunk_ = (size_t)cfg.sub_chunk * 16;
     return b;
 }
 
@@ -195,9 +205,16 @@ GpuBackend::~GpuBackend() {
 ChunkFn GpuBackend::gzip_chunk_fn() const {
     std::shared_ptr<const GpuBackend> self = shared_from_this();
     return [self](const uint8_t *data, size_t len) {
-        // the bound is twice the chunk: no zero fill of it, one copy of what is used
+        // the bound is twice the chunk: a per-thread buffer of that size,
+        // reused across calls (no zero fill, no fresh pages per chunk), and
+        // one copy of what is used into the result
+        thread_local std::unique_ptr<uint8_t[]> buf;
+        thread_local size_t buf_cap = 0;
         const size_t cap = gpz_compress_bound(self->ctx_, len);
-        std::unique_ptr<uint8_t[]> buf(new uint8_t[cap]);
+        if (cap > buf_cap) {
+            buf.reset(new uint8_t[cap]);
+            buf_cap = cap;
+        }
         size_t n = 0;
         int rc = gpz_compress_chunk(self->ctx_, data, len, buf.get(), cap, &n);
         if (rc != GPZ_OK) throw Error(Error::Kind::Io, std::string("gpu compress failed: ") + gpz_last_error());
@@ -211,7 +228,7 @@ Capability GpuBackend::supports(Algorithm a) const {
 
 std::unique_ptr<Compressor> GpuBackend::compressor(Algorithm a, Level) const {
     if (a != Algorithm::Gzip) throw Error(Error::Kind::UnsupportedAlgorithm, "backend `gpu` does not support this algorithm");
-    return std::make_unique<GpuGzipCompressor>(gzip_chunk_fn(), chunk_size_, max_in_flight_);
+    return std::make_unique<GpuGzipCompressor>(gzip_chunk_fn(), writer_chunk_, max_in_flight_);
 }
 
 const std::shared_ptr<GpuBackend> *LazyGpuBackend::try_get() const {

@@ -101,11 +101,11 @@ class HybridGzipCompressor : public Compressor {
 
 }  // namespace
 
-// Measured with bench.py --hybrid on binmix 256 MiB, 16 host cores: CPU only
-// 0.67 GB/s; the reference's one permit 1.27 GB/s (64 % of chunks on the
-// GPU); 4-8 permits with a 1 s wait for a permit 1.80-2.09 GB/s (97-100 %),
-// the lower figure with the 4-chunk warm-up kept, which spares small inputs
-// the GPU initialisation as in the reference.
+// Measured on binmix 1 GiB from host memory, 16 host cores, second pass (GPU
+// lanes warm): CPU only 0.81 GB/s; the reference's one permit 1.95 GB/s (63 %
+// of chunks on the GPU); 4 permits with a 1 s wait for a permit 4.60 GB/s
+// (99 %), 5.28 GB/s without the 4-chunk warm-up.  The warm-up is kept: it
+// spares small inputs the GPU initialisation, as in the reference.
 HybridTuning HybridTuning::retuned() {
     HybridTuning t;
     t.gpu_workers = 4;

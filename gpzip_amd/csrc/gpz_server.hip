@@ -329,6 +329,7 @@ int gpz_request_submit(gpz_server *s, const uint8_t *in, size_t len, uint64_t ta
     Request *r = owned.get();
     r->tag = tag;
     r->nbytes = len;
+    r->t_submit = clk::now();  // latency includes any wait for pool space
     const uint64_t cb = s->cfg.chunk_bytes;
     r->nchunks = len == 0 ? 1 : (uint32_t)((len + cb - 1) / cb);
     r->chunk_out.resize(r->nchunks);
@@ -352,7 +353,6 @@ int gpz_request_submit(gpz_server *s, const uint8_t *in, size_t len, uint64_t ta
     s->admit_ticket++;
     r->dev_len = need;
     r->admitted = true;
-    r->t_submit = clk::now();
     if (blocked) s->waited++;
     s->peak = std::max(s->peak, s->ws_bytes + s->inputs.used);
     lk.unlock();

@@ -90,6 +90,10 @@ class ParallelChunkedWriter : public Writer {
     size_t in_flight_ = 0;  // dispatched, result not yet posted
     std::map<uint32_t, std::vector<uint8_t>> pending_;
     std::deque<std::pair<uint32_t, std::shared_ptr<std::vector<uint8_t>>>> jobs_;
+    // input buffers whose chunk is done, kept for the next chunks: a fresh
+    // chunk-sized allocation per chunk costs the producer a page fault per
+    // 4 KiB while it copies, which caps a single producer near 2 GB/s
+    std::vector<std::vector<uint8_t>> spare_;
     std::mutex mu_;
     std::condition_variable cv_, cv_jobs_;
     std::vector<std::thread> workers_;
@@ -172,10 +176,17 @@ class GzipDecompressor : public Decompressor {
 class GpuBackend : public CodecBackend, public std::enable_shared_from_this<GpuBackend> {
   public:
     static constexpr const char *NAME = "gpu";
+    // lanes: concurrent GPU pipelines (0: max_in_flight, one per chunk in flight)
     static std::shared_ptr<GpuBackend> try_init(uint32_t lanes = 0);  // throws Error(Codec)
     ~GpuBackend() override;
+    // the member size (gpu.rs:160): the chunk function cuts its input into
+    // members of this size
     size_t chunk_size() const { return chunk_size_; }
     size_t max_in_flight() const { return max_in_flight_; }
+    // the compressor's writer chunk: a multiple of chunk_size, so the members
+    // (and bytes) are those of chunk_size chunks while each GPU call carries
+    // 16 members instead of one
+    size_t writer_chunk() const { return writer_chunk_; }
     gpz_ctx *context() const { return ctx_; }
     ChunkFn gzip_chunk_fn() const;
     const char *name() const override { return NAME; }
@@ -187,6 +198,7 @@ class GpuBackend : public CodecBackend, public std::enable_shared_from_this<GpuB
     gpz_ctx *ctx_ = nullptr;
     size_t chunk_size_ = 128 * 1024;
     size_t max_in_flight_ = 16;
+    size_t writer_chunk_ = 16 * 128 * 1024;
 };
 
 class LazyGpuBackend : public CodecBackend {

@@ -1,11 +1,13 @@
 // Tests of the C++ backend mirror (include/gpzip_backend.hpp), modelled on the
 // reference's scheduler tests (codec-cpu/src/parallel.rs:231-424).
 //   test_backend cpu        -- ParallelChunkedWriter with an identity framer, no GPU
-//   test_backend gpu FILE   -- GpuBackend compressor over FILE, output to stdout
+//   test_backend gpu FILE   -- GpuBackend compressor over FILE, output to stdout,
+//                              "seconds S" to stderr
 //   test_backend hybrid FILE GPU_WORKERS WARMUP [GPU_WAIT_MS]
 //                           -- HybridBackend (cli/src/hybrid.rs) over FILE,
 //                              stream to stdout, "gpu_chunks N cpu_chunks M
-//                              seconds S" to stderr
+//                              seconds S cold_seconds C" to stderr (two
+//                              passes: C includes first-use allocations)
 //   test_backend decode FILE -- GzipDecompressor over FILE, output to stdout
 //   test_backend tar OUT INPUT...    -- TarBuilder stream (uncompressed) to OUT
 //   test_backend targz OUT CHUNK BATCH INPUT...
@@ -279,17 +281,26 @@ int main(int argc, char **argv) {
         auto lazy = std::make_shared<LazyGpuBackend>((uint32_t)std::max<size_t>(4, t.gpu_workers));
         CHECK(lazy->try_get() != nullptr);  // initialised before the clock starts
         HybridBackend hy(std::make_shared<CpuBackend>(), lazy, t);
-        auto sink = std::make_unique<Sink>();
-        auto buf = sink->buf;
-        auto t0 = std::chrono::steady_clock::now();
-        {
-            auto w = hy.compressor(Algorithm::Gzip, Level{5})->wrap_writer(std::move(sink));
-            w->write_all(data.data(), data.size());
+        // two passes: the first pays the lanes' first-use allocations (device
+        // workspace, pinned staging); the second is the steady state
+        double sec[2];
+        std::shared_ptr<std::vector<uint8_t>> buf;
+        for (int pass = 0; pass < 2; pass++) {
+            stats->gpu_chunks = 0;
+            stats->cpu_chunks = 0;
+            auto sink = std::make_unique<Sink>();
+            buf = sink->buf;
+            buf->reserve(data.size() + (data.size() >> 4) + (1 << 20));
+            auto t0 = std::chrono::steady_clock::now();
+            {
+                auto w = hy.compressor(Algorithm::Gzip, Level{5})->wrap_writer(std::move(sink));
+                w->write_all(data.data(), data.size());
+            }
+            sec[pass] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
-        double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         fwrite(buf->data(), 1, buf->size(), stdout);
-        fprintf(stderr, "gpu_chunks %llu cpu_chunks %llu seconds %.6f\n", (unsigned long long)stats->gpu_chunks.load(),
-                (unsigned long long)stats->cpu_chunks.load(), sec);
+        fprintf(stderr, "gpu_chunks %llu cpu_chunks %llu seconds %.6f cold_seconds %.6f\n",
+                (unsigned long long)stats->gpu_chunks.load(), (unsigned long long)stats->cpu_chunks.load(), sec[1], sec[0]);
         return 0;
     }
     if (argc >= 4 && !strcmp(argv[1], "tar")) {
@@ -363,16 +374,28 @@ int main(int argc, char **argv) {
         CHECK(g != nullptr);
         CHECK((*g)->chunk_size() == 131072 && (*g)->max_in_flight() == 16);
         auto comp = lazy.compressor(Algorithm::Gzip, Level{5});
-        auto w = comp->wrap_writer(std::make_unique<FileSink>(stdout));
-        // odd write sizes, like a tar writer
-        size_t o = 0, step = 4099;
-        while (o < data.size()) {
-            size_t k = std::min(step, data.size() - o);
-            w->write_all(data.data() + o, k);
-            o += k;
-            step = step * 7 % 65521 + 1;
+        // two passes (the first pays the lanes' first-use allocations); the
+        // second, timed, goes to stdout
+        double sec[2];
+        for (int pass = 0; pass < 2; pass++) {
+            auto sink = std::make_unique<Sink>();
+            auto buf = sink->buf;
+            buf->reserve(data.size() + (data.size() >> 4) + (1 << 20));
+            auto t0 = std::chrono::steady_clock::now();
+            auto w = comp->wrap_writer(std::move(sink));
+            // odd write sizes, like a tar writer
+            size_t o = 0, step = 4099;
+            while (o < data.size()) {
+                size_t k = std::min(step, data.size() - o);
+                w->write_all(data.data() + o, k);
+                o += k;
+                step = step * 7 % 65521 + 1;
+            }
+            static_cast<ParallelChunkedWriter *>(w.get())->finalize();
+            sec[pass] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (pass == 1) fwrite(buf->data(), 1, buf->size(), stdout);
         }
-        static_cast<ParallelChunkedWriter *>(w.get())->finalize();
+        fprintf(stderr, "seconds %.6f cold_seconds %.6f\n", sec[1], sec[0]);
         return 0;
     }
     fprintf(stderr, "usage: test_backend cpu | gpu FILE\n");
