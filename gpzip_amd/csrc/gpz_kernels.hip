@@ -475,14 +475,14 @@ __global__ __launch_bounds__(BT) void k_seg_table(Work w) {
 // k_match_find: per-position tokens for one 4 KiB segment.
 //
 // The segment's window (32 KiB back, the segment, 258+ bytes ahead) is staged
-// in LDS.  The distinct hashes of the segment (its compact table) get dense ids
-// by rank in a 65536-bit presence map.  Then for k = 0..8 (the reference's
-// segment order, lz77_hash_lookup.wgsl:85-112; older segments only give
-// distances > 32768, rejected at :50) the compact table of segment s-k is
-// streamed in and every entry whose hash occurs in the segment stores its
-// (first, last) under the id, tagged with k; each position then tries newest,
-// then oldest while best < 16, stopping at 258 (:96-110).  65 KB of LDS: two
-// workgroups per CU.
+// in LDS.  Then for k = 0..8 (the reference's segment order,
+// lz77_hash_lookup.wgsl:85-112; older segments only give distances > 32768,
+// rejected at :50) the whole table of segment s-k -- 65536-bit presence map,
+// ranks, rank-ordered (first, last) entries -- is copied to LDS, each position
+// finds its hash's entry there (the reference's slot for (hash, segment)), and
+// tries newest, then oldest while best < 16, stopping at 258 (:96-110).  The
+// copy for s-k-1 travels while s-k is searched.  69 KB of LDS: two workgroups
+// per CU.
 
 // The walk summary of segment g from its tokens in LDS (tokL[0..L)); 1024
 // threads.  P is a ping-pong pair of 4096 words exit | count << 16.  Ends
@@ -593,22 +593,15 @@ __device__ __forceinline__ uint32_t ext_len(const uint32_t *win, uint32_t a, uin
     return len > lim ? lim : len;
 }
 
-// The distinct hashes of the query segment get dense ids by rank: a 65536-bit
-// presence map plus the number of set bits before each 32-bit word.
-__device__ __forceinline__ bool has_hash(const uint32_t *bits, uint32_t h) { return (bits[h >> 5] >> (h & 31)) & 1u; }
-__device__ __forceinline__ uint32_t hash_rank(const uint32_t *bits, const uint16_t *rnk, uint32_t h) {
-    return rnk[h >> 5] + __popc(bits[h >> 5] & ((1u << (h & 31)) - 1u));
-}
-
 // MINB: 8 gives two resident workgroups per CU (64 VGPRs, some spills), 4
 // gives one (128 VGPRs); the autotuner picks.
 template <int MINB>
 __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summary) {
     __shared__ __align__(16) uint32_t win[WIN_WORDS];
-    __shared__ uint32_t bits[2048];  // which hashes occur in the segment
-    __shared__ uint16_t rnk[2048];   // set bits before each word of `bits`
-    __shared__ uint32_t fl[SEG];     // per hash id: tag << 24 | first << 12 | last
-    __shared__ uint32_t tmp[32];
+    // the hash table of the segment being searched (own or back), copied whole:
+    // bitmap, ranks and its entries (layout of k_seg_table), +4 words so an
+    // absent hash's entry index (at most 4096) reads inside the array
+    __shared__ __align__(16) uint32_t tab[TAB_WORDS + 4];
     __shared__ uint32_t kq[K1_T];    // per wave: 64-entry queue of pending candidates
     // Workgroup votes, one word each: [k] back segment k, [15] the walk summary.  A
     // wave with a set predicate writes 1 before a barrier; everyone reads after it.
@@ -617,6 +610,24 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
     __shared__ uint32_t kvote[16];
     const uint32_t tid = threadIdx.x;
     const uint32_t G = uni(w.counts->nseg);
+    // segment g-k's table -> tab through the direct global-to-LDS path (no data
+    // registers): the bitmap and ranks (TAB_ENT words) and the segment's D
+    // entries, 16 bytes per lane, lane i of a wave at its base + 16 i
+    auto copy_table = [&](uint32_t gk) {
+        uint32_t t = tid;
+        asm volatile("" : "+v"(t));  // lane offsets recomputed per copy, not held
+        const uint32_t *T = w.segtab + (uint64_t)gk * TAB_WORDS;
+        const uint32_t n16 = (TAB_ENT + uni(w.seg_ntab[gk]) + 3u) >> 2;
+        static_assert(TAB_WORDS <= 8 * K1_T, "two 16-byte copies per lane");
+#pragma unroll
+        for (uint32_t r = 0; r < 2; r++) {
+            const uint32_t u = r * K1_T + t;
+            const uint32_t wb = __builtin_amdgcn_readfirstlane(r * K1_T + (t & ~63u));
+            if (u < n16)
+                __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(T + 4u * u),
+                                                 (void __attribute__((address_space(3))) *)(tab + 4u * wb), 16, 0, 0);
+        }
+    };
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
         const MemberDesc md = load_member(w.mem, uni(w.seg_mem[g]));
         const uint32_t n = md.n, s = g - md.seg_base;
@@ -627,6 +638,7 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
         const uint32_t nb = WE - WB;
         const uint32_t kmax = min(s, 8u);
 
+        copy_table(g);  // the own table (k = 0) travels while the window loads
         // ---- window -> LDS (zero beyond the member end)
         {
             const uint8_t *src = w.in + md.in_off + WB;
@@ -648,22 +660,14 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                 }
                 reinterpret_cast<uint4 *>(win)[v] = q;
             }
-        }
-        {
-            // the segment's own hash bitmap and ranks (2 words / 2 ranks per thread)
-            const uint32_t *T = w.segtab + (uint64_t)g * TAB_WORDS;
-            const uint2 bw = reinterpret_cast<const uint2 *>(T + TAB_BITS)[tid];
-            bits[2 * tid] = bw.x;
-            bits[2 * tid + 1] = bw.y;
-            reinterpret_cast<uint32_t *>(rnk)[tid] = T[TAB_RANK + tid];
             if (tid < 16) kvote[tid] = 0;
         }
         __syncthreads();
 
-        // ---- own positions: hash -> id
+        // ---- own positions: their hashes, two 16-bit hashes per register
         const uint32_t x0 = tid * 4u;
         const uint32_t lx = S - WB + x0;
-        uint32_t qid[4];
+        uint32_t hh0 = 0, hh1 = 0;
         uint32_t hvm = 0;  // bit i: position x0+i can start a match (p + 2 < n)
         const uint32_t own_lo = win[lx >> 2], own_hi = win[(lx >> 2) + 1];  // bytes x0 .. x0+7
         const uint32_t own_w2 = win[(lx >> 2) + 2];                          // bytes x0+8 .. x0+11
@@ -675,70 +679,38 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                 const bool hv = x < L && S + x + 2u < n;
                 const uint32_t h = hash3((uint32_t)(vv >> (8 * i)) & 0xFF, (uint32_t)(vv >> (8 * i + 8)) & 0xFF,
                                          (uint32_t)(vv >> (8 * i + 16)) & 0xFF);
-                qid[i] = hv ? hash_rank(bits, rnk, h) : 0;
+                if (i < 2) hh0 |= h << (16 * i);
+                else hh1 |= h << (16 * (i - 2));
                 hvm |= (uint32_t)hv << i;
             }
         }
 
         uint32_t st[4] = {0, 0, 0, 0};  // best len << 16 | dist per position
         for (uint32_t k = 0; k <= kmax; k++) {
-            // segment s-k's table: every hash that also occurs in this segment
-            // stores its (first, last) under its id here, tagged with k
+            // tab holds segment s-k's table once this wave's copies have landed
+            // and every wave has passed the barrier; from k = 1 every wave has
+            // also voted on k - 1
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            __syncthreads();
+            if (k && !kvote[k - 1]) break;
+            // recomputed from the packed hashes each step: hoisted out of the
+            // loop, the per-position addresses and masks would spill
+            asm volatile("" : "+v"(hh0), "+v"(hh1));
+            // each position's (first, last) in segment s-k, or EMPTY when its
+            // hash does not occur there (the reference's empty slot)
+            uint32_t fv[4], pd[4] = {0, 0, 0, 0};
             {
-                const uint32_t *T = w.segtab + (uint64_t)(g - k) * TAB_WORDS;
-                if (k == 0) {
-                    // own hashes: ids are ranks, the entries are already in id order
-                    const uint32_t D = uni(w.seg_ntab[g]);
+                const uint16_t *rk = reinterpret_cast<const uint16_t *>(tab + TAB_RANK);
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const uint32_t i = tid + r * K1_T;
-                        if (i < D) fl[i] = T[TAB_ENT + i];
-                    }
-                } else {
-                    const uint2 bk = reinterpret_cast<const uint2 *>(T + TAB_BITS)[tid];
-                    const uint32_t rkk = T[TAB_RANK + tid];
-                    const uint32_t o0 = bits[2 * tid], o1 = bits[2 * tid + 1];
-                    const uint32_t I0 = o0 & bk.x, I1 = o1 & bk.y;
-                    // the wave's common hashes as (id, entry index) pairs in its
-                    // queue, then one entry load per lane, 64 at a time
-                    const uint32_t c = __popc(I0) + __popc(I1);
-                    const uint32_t incl = wave_scan_incl(c);
-                    const uint32_t tot = __builtin_amdgcn_readlane(incl, 63);
-                    if (tot) {
-                        const uint32_t lane = tid & 63u;
-                        uint32_t *q = kq + (tid & ~63u);
-                        const uint32_t ro0 = rnk[2 * tid], ro1 = rnk[2 * tid + 1];
-                        const uint32_t tag = k << 24;
-                        // the lane's pairs have queue indices [incl - c, incl): each
-                        // round of 64 takes up where the previous one stopped
-                        uint32_t J0 = I0, J1 = I1, idx = incl - c;
-                        for (uint32_t r0 = 0; r0 < tot; r0 += 64) {
-                            while ((J0 | J1) && idx < r0 + 64u) {
-                                uint32_t id, ent;
-                                if (J0) {
-                                    const uint32_t m = (1u << __builtin_ctz(J0)) - 1u;
-                                    id = ro0 + __popc(o0 & m);
-                                    ent = (rkk & 0xFFFFu) + __popc(bk.x & m);
-                                    J0 &= J0 - 1u;
-                                } else {
-                                    const uint32_t m = (1u << __builtin_ctz(J1)) - 1u;
-                                    id = ro1 + __popc(o1 & m);
-                                    ent = (rkk >> 16) + __popc(bk.y & m);
-                                    J1 &= J1 - 1u;
-                                }
-                                q[idx - r0] = id | (ent << 12);
-                                idx++;
-                            }
-                            wave_lds_sync();
-                            if (r0 + lane < tot) {
-                                const uint32_t it = q[lane];
-                                fl[it & 0xFFFu] = tag | T[TAB_ENT + (it >> 12)];
-                            }
-                            wave_lds_sync();
-                        }
-                    }
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t h = ((i < 2 ? hh0 : hh1) >> (16 * (i & 1))) & 0xFFFFu;
+                    const uint32_t wd = tab[TAB_BITS + (h >> 5)];
+                    const uint32_t below = wd & ((1u << (h & 31u)) - 1u);
+                    const uint32_t e = tab[TAB_ENT + rk[h >> 5] + __popc(below)];
+                    fv[i] = (((hvm >> i) & 1u) && ((wd >> (h & 31u)) & 1u) && (st[i] >> 16) < MAX_MATCH) ? e : EMPTY;
                 }
             }
+            // every wave has its entries: tab is free for the next table
             __syncthreads();
             // evaluate segment s-k in the reference order per position: newest,
             // then oldest while best < 16.  Pass 1 resolves every candidate over
@@ -749,17 +721,13 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
             // pending candidate.  Pass 2 extends the pending candidates of the
             // wave densely, 64 at a time, through a wave-private LDS queue.
             const uint32_t base = (s - k) << SEG_LOG2;
-            uint32_t fv[4], pd[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                fv[i] = (((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH) ? fl[qid[i]] : EMPTY;
             auto pass1 = [&](auto fast) {
             constexpr bool FAST = decltype(fast)::value;
 #pragma unroll 1
             for (uint32_t i = 0; i < 4; i++) {
                 const uint32_t v = fv[0];
                 uint32_t cs = st[0], pend = 0;
-                if ((v >> 24) == k) {
+                if (v != EMPTY) {
                     const uint32_t p = S + x0 + i;
                     const uint32_t lim = FAST ? MAX_MATCH : min(n - p, MAX_MATCH);
                     const uint32_t own0 = __builtin_amdgcn_alignbyte(own_hi, own_lo, i);
@@ -832,12 +800,13 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                     wave_lds_sync();
                 }
             }
+            // the next table travels while the waves vote and meet (issued here,
+            // after both passes, it holds no registers through them)
+            if (k < kmax) copy_table(g - k - 1);
             int active = 0;
 #pragma unroll
             for (int i = 0; i < 4; i++) active |= ((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH;
             if (__ballot(active) && (tid & 63u) == 0) kvote[k] = 1u;
-            __syncthreads();
-            if (!kvote[k]) break;
         }
 
         // ---- tokens
@@ -858,7 +827,7 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
         if (do_summary) {
             // the segment's walk summary from the same tokens, in the LDS the
             // match search no longer needs: tokens over the window, one
-            // exit | count array after them, the other over fl
+            // exit | count array after them, the other over tab
             __syncthreads();
             uint32_t *tokL = win;
             int anym = 0;
@@ -869,7 +838,7 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
             }
             if (__ballot(anym) && (tid & 63u) == 0) kvote[15] = 1u;
             __syncthreads();
-            seg_summary(w, g, L, E == n, kvote[15] != 0, tokL, win + SEG, fl);
+            seg_summary(w, g, L, E == n, kvote[15] != 0, tokL, win + SEG, tab);
         }
         __syncthreads();
     }
