@@ -669,10 +669,10 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
         const uint32_t lx = S - WB + x0;
         uint32_t hh0 = 0, hh1 = 0;
         uint32_t hvm = 0;  // bit i: position x0+i can start a match (p + 2 < n)
-        const uint32_t own_lo = win[lx >> 2], own_hi = win[(lx >> 2) + 1];  // bytes x0 .. x0+7
-        const uint32_t own_w2 = win[(lx >> 2) + 2];                          // bytes x0+8 .. x0+11
+        const uint32_t own_lo0 = win[lx >> 2], own_hi0 = win[(lx >> 2) + 1];  // bytes x0 .. x0+7
+        const uint32_t own_w20 = win[(lx >> 2) + 2];                           // bytes x0+8 .. x0+11
         {
-            const uint64_t vv = (uint64_t)own_lo | ((uint64_t)own_hi << 32);
+            const uint64_t vv = (uint64_t)own_lo0 | ((uint64_t)own_hi0 << 32);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint32_t x = x0 + i;
@@ -693,9 +693,16 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
             __syncthreads();
             if (k && !kvote[k - 1]) break;
-            // recomputed from the packed hashes each step: hoisted out of the
-            // loop, the per-position addresses and masks would spill
+            // Values the loop does not change pass through empty asm operands
+            // each step, so what is derived from them (lookup addresses and
+            // masks from the packed hashes; byte-aligned own words and limits
+            // per position of the unrolled passes) is recomputed per step:
+            // hoisted out of the loop, it would be held across it and spill.
             asm volatile("" : "+v"(hh0), "+v"(hh1));
+            uint32_t own_lo = own_lo0, own_hi = own_hi0, own_w2 = own_w20;
+            asm volatile("" : "+v"(own_lo), "+v"(own_hi), "+v"(own_w2));
+            uint32_t xk = x0;
+            asm volatile("" : "+v"(xk));
             // each position's (first, last) in segment s-k, or EMPTY when its
             // hash does not occur there (the reference's empty slot)
             uint32_t fv[4], pd[4] = {0, 0, 0, 0};
@@ -714,8 +721,7 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
             __syncthreads();
             // evaluate segment s-k in the reference order per position: newest,
             // then oldest while best < 16.  Pass 1 resolves every candidate over
-            // its first 16 bytes (position by position; the arrays rotate so the
-            // loop body is not unrolled).  A candidate whose 16 bytes agree is
+            // its first 16 bytes, position by position.  A candidate whose 16 bytes agree is
             // left pending: best is then >= 16 whatever its length, so the
             // oldest rule is already decided, and a position has at most one
             // pending candidate.  Pass 2 extends the pending candidates of the
@@ -723,12 +729,12 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
             const uint32_t base = (s - k) << SEG_LOG2;
             auto pass1 = [&](auto fast) {
             constexpr bool FAST = decltype(fast)::value;
-#pragma unroll 1
+#pragma unroll
             for (uint32_t i = 0; i < 4; i++) {
-                const uint32_t v = fv[0];
-                uint32_t cs = st[0], pend = 0;
+                const uint32_t v = fv[i];
+                uint32_t cs = st[i], pend = 0;
                 if (v != EMPTY) {
-                    const uint32_t p = S + x0 + i;
+                    const uint32_t p = S + xk + i;
                     const uint32_t lim = FAST ? MAX_MATCH : min(n - p, MAX_MATCH);
                     const uint32_t own0 = __builtin_amdgcn_alignbyte(own_hi, own_lo, i);
                     const uint32_t own1 = __builtin_amdgcn_alignbyte(own_w2, own_hi, i);
@@ -748,9 +754,8 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                         }
                     }
                 }
-                fv[0] = fv[1]; fv[1] = fv[2]; fv[2] = fv[3];
-                st[0] = st[1]; st[1] = st[2]; st[2] = st[3]; st[3] = cs;
-                pd[0] = pd[1]; pd[1] = pd[2]; pd[2] = pd[3]; pd[3] = pend;
+                st[i] = cs;
+                pd[i] = pend;
             }
             };
             // interior segments (every match limit is 258) and back segments 1 .. 7
