@@ -515,11 +515,14 @@ __device__ __forceinline__ void seg_summary(const Work &w, uint32_t g, uint32_t 
         if (valid) P0[x] = ex | (c << 16);
     }
     __syncthreads();
-    // level 2: pointer doubling over the block exits.  Every exit lies in a
-    // later block, so 64 hops (6 rounds) reach the segment end or the
-    // absorbing last position (its own exit, count 0).
+    // level 2: pointer doubling over the block exits, three rounds (8 hops
+    // per pointer); then each entry x < 258 follows the remaining pointers.
+    // Every exit lies in a later block, so at most 8 more hops reach the
+    // segment end or the absorbing last position (its own exit, count 0).
+    // Three rounds over all positions and a short chase on 258 lanes cost
+    // fewer instructions than six rounds.
     uint32_t *cur = P0, *nxt = P1;
-    for (int r = 0; r < 6; r++) {
+    for (int r = 0; r < 3; r++) {
         for (uint32_t x = tid; x < L; x += 1024) {
             const uint32_t v = cur[x], y = v & 0xFFFFu;
             uint32_t nv = v;
@@ -534,7 +537,17 @@ __device__ __forceinline__ void seg_summary(const Work &w, uint32_t g, uint32_t 
         cur = nxt;
         nxt = t;
     }
-    for (uint32_t x = tid; x < L && x < SPAN; x += 1024) out[x] = cur[x];
+    for (uint32_t x = tid; x < L && x < SPAN; x += 1024) {
+        const uint32_t v = cur[x];
+        uint32_t y = v & 0xFFFFu, c = v >> 16;
+        for (int h = 0; h < 64 && y < L; h++) {
+            const uint32_t v2 = cur[y], y2 = v2 & 0xFFFFu;
+            if (y2 == y) break;  // the absorbing last position
+            c += v2 >> 16;
+            y = y2;
+        }
+        out[x] = y | (c << 16);
+    }
 }
 
 constexpr int K1_T = 1024;
