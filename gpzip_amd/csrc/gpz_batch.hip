@@ -99,6 +99,10 @@ int lane_step_resident(gpz_ctx *ctx, Lane *ln, const Plan &pl, const void *d_in)
     HIPCHK(hipSetDevice(ctx_device(ctx)));
     if ((rc = ln->wsp.upload_plan(pl, ln->st))) return rc;
     Work w = ln->wsp.work;
+    if (pl.device_tables) {
+        w.cdesc = (const uint64_t *)(ln->wsp.base + ln->wsp.lay.cdesc);
+        w.sub = pl.sub;
+    }
     w.in = (const uint8_t *)d_in;
     w.gzip = 1;
     w.strict = ctx_walk_rule(ctx);
@@ -179,7 +183,7 @@ int gpz_batch_submit(gpz_ctx *ctx, const uint8_t *base, const gpz_desc *descs, s
     for (size_t i = 0; i < n; i++) lens[i] = descs[i].len;
     auto b = new gpz_batch();
     b->ctx = ctx;
-    int rc = build_plan(b->pl, lens.data(), nullptr, n, ctx_sub_chunk(ctx), 1);
+    int rc = build_plan(b->pl, lens.data(), nullptr, n, ctx_sub_chunk(ctx), 1, true);
     if (rc) {
         delete b;
         return rc;
@@ -268,7 +272,7 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
     auto p = new gpz_plan();
     p->ctx = ctx;
     p->flags = flags;
-    int rc = build_plan(p->pl, lens.data(), offs.data(), n, ctx_sub_chunk(ctx), 1);
+    int rc = build_plan(p->pl, lens.data(), offs.data(), n, ctx_sub_chunk(ctx), 1, true);
     if (!rc) rc = p->lane.init(ctx_device(ctx));
     if (!rc) {
         const Plan &pl = p->pl;
@@ -447,6 +451,8 @@ int gpz_plan_run(gpz_plan *p, const void *d_in, void *d_out) {
     Lane &ln = p->lane;
     HIPCHK(hipSetDevice(ctx_device(p->ctx)));
     Work w = ln.wsp.work;
+    w.cdesc = (const uint64_t *)(ln.wsp.base + ln.wsp.lay.cdesc);  // tables built per run on the device
+    w.sub = p->pl.sub;
     w.in = (const uint8_t *)d_in;
     w.out = (uint8_t *)d_out;
     w.gzip = 1;

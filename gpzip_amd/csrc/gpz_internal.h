@@ -73,7 +73,11 @@ struct Work {
                                 // lookahead (the opt-in ratio mode)
     uint32_t gzip;              // 1: gzip members, 0: raw DEFLATE (encode_block)
     uint32_t window;            // 32768
-    uint32_t pad;
+    uint32_t sub;               // member size (k_plan_batch)
+    // chunk descriptors the device builds the tables above from (k_plan_batch),
+    // or null when the host uploads the tables: [0] the chunk count C, [1]
+    // unused, then C pairs {byte offset of the chunk in `in`, length}
+    const uint64_t *cdesc;
 };
 
 }  // namespace gpz

@@ -1811,6 +1811,79 @@ __global__ __launch_bounds__(256) void k_plan_one(Work w, uint32_t len, uint32_t
     }
 }
 
+// k_plan_batch: the member and segment tables of a batch, built on the device
+// from its chunk descriptors (w.cdesc) -- the host's build_plan in closed form
+// per chunk: ceil(len / sub) members (one when len <= sub), every member but
+// the last full.  The descriptors are the only per-batch upload, so a
+// captured graph that starts with this kernel serves any batch within the
+// workspace's capacity.  One workgroup: chunks in tiles of 1024, a scan of
+// their member, segment and position counts per tile, then one thread per
+// member of the tile.
+__global__ __launch_bounds__(1024) void k_plan_batch(Work w) {
+    __shared__ uint32_t tmp[16];
+    __shared__ uint32_t t_m0[1024], t_g0[1024], t_p0[1024];  // tile-relative bases per chunk
+    MemberDesc *mem = const_cast<MemberDesc *>(w.mem);
+    uint32_t *seg_mem = const_cast<uint32_t *>(w.seg_mem);
+    uint32_t *chunk_mem0 = const_cast<uint32_t *>(w.chunk_mem0);
+    Counts *counts = const_cast<Counts *>(w.counts);
+    const uint64_t *cd = w.cdesc;
+    const uint32_t C = (uint32_t)cd[0];
+    const uint32_t sub = w.sub, tid = threadIdx.x;
+    const uint32_t spm = (sub + SEG - 1) >> SEG_LOG2;        // segments of a full member
+    const uint32_t pstep = (sub + POS_ALIGN - 1) / POS_ALIGN;  // positions of a full member, / 64
+    uint32_t cm = 0, cg = 0;
+    uint64_t cp = 0;  // members, segments and positions / 64 before the tile
+    for (uint32_t c0 = 0; c0 < C; c0 += 1024) {
+        const uint32_t nc = min(1024u, C - c0), c = c0 + tid;
+        uint32_t nm = 0, ng = 0, np = 0;
+        if (tid < nc) {
+            const uint64_t len = cd[3 + 2 * (uint64_t)c];
+            const uint64_t m = len <= sub ? 1 : (len + sub - 1) / sub;
+            const uint32_t nlast = (uint32_t)(len - (m - 1) * sub);
+            nm = (uint32_t)m;
+            ng = (nm - 1) * spm + ((nlast + SEG - 1) >> SEG_LOG2);
+            np = (nm - 1) * pstep + (nlast + POS_ALIGN - 1) / POS_ALIGN;
+        }
+        uint32_t tm, tg, tp;
+        t_m0[tid] = block_scan_dpp<1024>(nm, tmp, &tm);
+        t_g0[tid] = block_scan_dpp<1024>(ng, tmp, &tg);
+        t_p0[tid] = block_scan_dpp<1024>(np, tmp, &tp);
+        if (tid < nc) chunk_mem0[c] = cm + t_m0[tid];
+        __syncthreads();
+        for (uint32_t j = tid; j < tm; j += 1024) {
+            // the tile's chunk of member j: the last t with t_m0[t] <= j (every
+            // chunk has a member, so the bases rise strictly)
+            uint32_t lo = 0, hi = nc;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (t_m0[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            const uint32_t cc = c0 + lo, i = j - t_m0[lo];
+            const uint64_t off = cd[2 + 2 * (uint64_t)cc], len = cd[3 + 2 * (uint64_t)cc];
+            const uint64_t m = len <= sub ? 1 : (len + sub - 1) / sub;
+            MemberDesc d;
+            d.n = i + 1 == m ? (uint32_t)(len - (m - 1) * sub) : sub;
+            d.in_off = off + (uint64_t)i * sub;
+            d.pos_base = (cp + t_p0[lo] + (uint64_t)i * pstep) * POS_ALIGN;
+            d.seg_base = cg + t_g0[lo] + i * spm;
+            d.nseg = (d.n + SEG - 1) >> SEG_LOG2;
+            d.chunk = cc;
+            const uint32_t mi = cm + j;
+            mem[mi] = d;
+            for (uint32_t q = 0; q < d.nseg; q++) seg_mem[d.seg_base + q] = mi;
+        }
+        cm += tm;
+        cg += tg;
+        cp += tp;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        chunk_mem0[C] = cm;
+        *counts = Counts{cg, cm, C, 0};
+    }
+}
+
 }  // namespace gpz
 
 // ---------------------------------------------------------------------------
@@ -1879,6 +1952,9 @@ void launch_huffman(const Work &w, uint32_t grid, hipStream_t st) {
 void launch_scan(const Work &w, hipStream_t st) { hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, w); }
 void launch_plan_one(const Work &w, uint32_t len, uint32_t sub, hipStream_t st) {
     hipLaunchKernelGGL(k_plan_one, dim3(1), dim3(256), 0, st, w, len, sub);
+}
+void launch_plan_batch(const Work &w, hipStream_t st) {
+    hipLaunchKernelGGL(k_plan_batch, dim3(1), dim3(1024), 0, st, w);
 }
 void launch_emit(const Work &w, uint32_t grid, int variant, hipStream_t st) {
     if (variant == 1)

@@ -22,26 +22,31 @@ const char *last_error();
 struct DevBuf;
 struct HostBuf;
 
-// Chunk list expanded to members and segments.
+// Chunk list expanded to members and segments.  With device_tables the
+// member and segment tables are not built here: only the totals (which size
+// the workspace) and the chunk descriptors, from which k_plan_batch builds
+// the tables on the device.
 struct Plan {
     std::vector<MemberDesc> mem;
     std::vector<uint32_t> seg_mem;
     std::vector<uint32_t> chunk_mem0;     // C + 1 entries
     std::vector<uint64_t> chunk_dev_off;  // chunk offsets in the device input buffer
+    std::vector<uint64_t> chunk_len;
     uint64_t P = 0, in_end = 0, in_bytes = 0, out_cap = 0;
-    uint32_t G = 0, M = 0, C = 0;
+    uint32_t G = 0, M = 0, C = 0, sub = 0;
     int gzip = 1;
+    bool device_tables = false;
 };
 
 int build_plan(Plan &pl, const uint64_t *lens, const uint64_t *dev_off, size_t C, uint32_t sub_chunk,
-               int gzip);
+               int gzip, bool device_tables = false);
 
 struct Caps {
     uint64_t P = 0, G = 0, M = 0, C = 0, out = 0;
 };
 
 struct Layout {
-    uint64_t mem, seg_mem, chunk_mem0, counts, desc_end;
+    uint64_t mem, seg_mem, chunk_mem0, counts, cdesc, desc_end;
     uint64_t tok, segtab, seg_ntab, wsum, seg_crc, seg_entry, seg_tokbase, walked, hist, ntok, codes, prefix, prefix_bits;
     uint64_t mem_bytes, mem_crc, mem_off, chunk_off, chunk_len, total, out, end;
 };

@@ -38,6 +38,7 @@ void launch_huffman(const Work &w, uint32_t grid, hipStream_t st);
 void launch_scan(const Work &w, hipStream_t st);
 void launch_emit(const Work &w, uint32_t grid, int variant, hipStream_t st);
 void launch_plan_one(const Work &w, uint32_t len, uint32_t sub, hipStream_t st);
+void launch_plan_batch(const Work &w, hipStream_t st);
 
 static thread_local char g_err[512];
 
@@ -100,13 +101,47 @@ HostBuf::~HostBuf() {
 // plan: chunk list -> members -> segments
 
 int build_plan(Plan &pl, const uint64_t *lens, const uint64_t *dev_off, size_t C, uint32_t sub_chunk,
-               int gzip) {
+               int gzip, bool device_tables) {
     pl = Plan();
     if (sub_chunk == 0) return fail(GPZ_E_INVAL, "sub_chunk must be > 0");
+    if (sub_chunk > (16u << 20)) return fail(GPZ_E_INVAL, "members above 16 MiB");
     pl.C = (uint32_t)C;
+    pl.sub = sub_chunk;
+    pl.device_tables = device_tables;
     pl.chunk_dev_off.resize(C);
-    pl.chunk_mem0.resize(C + 1);
+    pl.chunk_len.assign(lens, lens + C);
     uint64_t in_off = 0, pos = 0, out = 0;
+    if (device_tables) {
+        // totals in closed form per chunk; k_plan_batch writes the tables
+        const uint64_t spm = (sub_chunk + SEG - 1) >> SEG_LOG2, pstep = round_up(sub_chunk, POS_ALIGN);
+        uint64_t M = 0, G = 0;
+        for (size_t c = 0; c < C; c++) {
+            const uint64_t len = lens[c];
+            if (dev_off) {
+                if (dev_off[c] % 16) return fail(GPZ_E_INVAL, "chunk %zu offset not 16-byte aligned", c);
+                pl.chunk_dev_off[c] = dev_off[c];
+            } else {
+                pl.chunk_dev_off[c] = in_off;
+                in_off += round_up(len, IN_ALIGN);
+            }
+            pl.in_end = std::max<uint64_t>(pl.in_end, pl.chunk_dev_off[c] + len);
+            const uint64_t nm = len <= sub_chunk ? 1 : (len + sub_chunk - 1) / sub_chunk;
+            const uint64_t nlast = len - (nm - 1) * sub_chunk;
+            M += nm;
+            G += (nm - 1) * spm + ((nlast + SEG - 1) >> SEG_LOG2);
+            pos += (nm - 1) * pstep + round_up(nlast, POS_ALIGN);
+            out += 2 * len + nm * (1024 + (gzip ? 18 : 0));
+        }
+        if (M >= (1ull << 32) || G >= (1ull << 32)) return fail(GPZ_E_INVAL, "batch too large");
+        pl.M = (uint32_t)M;
+        pl.G = (uint32_t)G;
+        pl.P = pos + POS_ALIGN;
+        pl.out_cap = round_up(out, 256) + 256;
+        pl.in_bytes = round_up(pl.in_end, IN_ALIGN) + IN_ALIGN;
+        pl.gzip = gzip;
+        return GPZ_OK;
+    }
+    pl.chunk_mem0.resize(C + 1);
     for (size_t c = 0; c < C; c++) {
         uint64_t len = lens[c];
         if (dev_off) {
@@ -163,6 +198,7 @@ Layout layout_for(const Caps &c) {
     L.seg_mem = carve(cur, c.G * 4);
     L.chunk_mem0 = carve(cur, (c.C + 1) * 4);
     L.counts = carve(cur, sizeof(Counts));
+    L.cdesc = carve(cur, 16 + c.C * 16);
     L.desc_end = cur;
     L.tok = carve(cur, c.P * 4);
     L.segtab = carve(cur, (uint64_t)c.G * TAB_WORDS * 4);
@@ -262,6 +298,18 @@ int Workspace::bind(const Caps &c, void *mem, uint64_t bytes) {
 // Copy the plan's descriptor tables into pinned staging and enqueue the upload.
 int Workspace::upload_plan(const Plan &pl, hipStream_t st) {
     uint8_t *h = (uint8_t *)hdesc.p;
+    if (pl.device_tables) {
+        // the chunk descriptors only; k_plan_batch derives the tables
+        uint64_t *d = (uint64_t *)(h + lay.cdesc);
+        d[0] = pl.C;
+        d[1] = 0;
+        for (uint32_t c = 0; c < pl.C; c++) {
+            d[2 + 2 * (uint64_t)c] = pl.chunk_dev_off[c];
+            d[3 + 2 * (uint64_t)c] = pl.chunk_len[c];
+        }
+        HIPCHK(hipMemcpyAsync(base + lay.cdesc, d, 16 + (uint64_t)pl.C * 16, hipMemcpyHostToDevice, st));
+        return GPZ_OK;
+    }
     memcpy(h + lay.mem, pl.mem.data(), pl.mem.size() * sizeof(MemberDesc));
     memcpy(h + lay.seg_mem, pl.seg_mem.data(), pl.seg_mem.size() * 4);
     memcpy(h + lay.chunk_mem0, pl.chunk_mem0.data(), pl.chunk_mem0.size() * 4);
@@ -331,6 +379,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
             if (ev) hipEventRecord(ev[4], st);
             break;
         default:  // MODE_FULL
+            if (w.cdesc) launch_plan_batch(w, st);  // tables from the chunk descriptors
             launch_seg_table(w, g.k0, g.k0v, st);
             if (ev) hipEventRecord(ev[5], st);
             if (w.strict == 2) {

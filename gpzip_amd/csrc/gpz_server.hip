@@ -182,7 +182,7 @@ struct gpz_server {
                 offs[i] = step[i].off;
             }
             Plan pl;
-            int rc = build_plan(pl, lens.data(), offs.data(), step.size(), ctx_sub_chunk(ctx), 1);
+            int rc = build_plan(pl, lens.data(), offs.data(), step.size(), ctx_sub_chunk(ctx), 1, true);
             if (!rc) rc = lane_step_resident(ctx, ln, pl, pool);
             const uint8_t *res = nullptr;
             if (!rc) rc = lane_batch_collect_pinned(ln, pl, &res, off, len);

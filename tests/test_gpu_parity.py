@@ -165,6 +165,24 @@ def test_batch_equals_plain(ctx):
     assert ctx.batch(chunks) == res
 
 
+def test_device_plan_across_tiles_and_smaller_batches(ctx):
+    """The batched form builds its member and segment tables on the device
+    from the chunk descriptors (k_plan_batch, tiles of 1024 chunks): 1,500
+    chunks of 0-9000 bytes (two tiles, empty chunks among them), then a
+    3-chunk batch replayed on the same lane, each equal to the plain form."""
+    rng = np.random.default_rng(77)
+    sizes = rng.integers(0, 9000, 1500)
+    sizes[[0, 1023, 1024, 1499]] = [0, 131073, 0, 4096]
+    kinds = ["text", "binmix", "log"]
+    chunks = [gpzip_amd.synth(kinds[i % 3], max(int(s), 1), 700 + i)[:int(s)] for i, s in enumerate(sizes)]
+    say("device plan", len(chunks))
+    res = ctx.batch(chunks)
+    for i, (c, r) in enumerate(zip(chunks, res)):
+        assert r == ctx.compress_chunk(c), f"chunk {i} ({c.size} bytes)"
+    small = [chunks[1023], chunks[5], chunks[1024]]
+    assert ctx.batch(small) == [res[1023], res[5], res[1024]]
+
+
 def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     """The autotuner returns a valid record; every launch shape it can pick
     (and both match-find occupancy variants) gives the same bytes."""
