@@ -272,7 +272,10 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
     auto p = new gpz_plan();
     p->ctx = ctx;
     p->flags = flags;
-    int rc = build_plan(p->pl, lens.data(), offs.data(), n, ctx_sub_chunk(ctx), 1, true);
+    // graph plans (the batched form) build their tables on the device from the
+    // descriptors at each replay; direct-launch plans (the plain form's shape)
+    // keep the tables uploaded once here, as gpz_compress_chunk uploads its own
+    int rc = build_plan(p->pl, lens.data(), offs.data(), n, ctx_sub_chunk(ctx), 1, (flags & GPZ_PLAN_GRAPH) != 0);
     if (!rc) rc = p->lane.init(ctx_device(ctx));
     if (!rc) {
         const Plan &pl = p->pl;
@@ -451,8 +454,10 @@ int gpz_plan_run(gpz_plan *p, const void *d_in, void *d_out) {
     Lane &ln = p->lane;
     HIPCHK(hipSetDevice(ctx_device(p->ctx)));
     Work w = ln.wsp.work;
-    w.cdesc = (const uint64_t *)(ln.wsp.base + ln.wsp.lay.cdesc);  // tables built per run on the device
-    w.sub = p->pl.sub;
+    if (p->pl.device_tables) {
+        w.cdesc = (const uint64_t *)(ln.wsp.base + ln.wsp.lay.cdesc);  // tables built per run on the device
+        w.sub = p->pl.sub;
+    }
     w.in = (const uint8_t *)d_in;
     w.out = (uint8_t *)d_out;
     w.gzip = 1;

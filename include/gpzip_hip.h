@@ -222,7 +222,10 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
 /* Plan for a fixed list of chunks laid out in a device buffer: workspace is
  * allocated once; gpz_plan_run may then be called repeatedly.  Chunk offsets
  * must be multiples of 16.  d_in must have 256 readable bytes past the last
- * chunk. */
+ * chunk.  With GPZ_PLAN_GRAPH the run is one graph replay whose first kernel
+ * builds the member and segment tables on the device from the chunk
+ * descriptors (as gpz_batch_* and the serving form do); without it the
+ * tables are built here once and the kernels are launched directly. */
 int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flags,
                     const gpz_tune *tune, gpz_plan **out);
 /* Output capacity the plan needs in d_out. */
