@@ -509,7 +509,12 @@ int lane_run(gpz_ctx *ctx, Lane *ln, const Plan &pl, int mode, const uint8_t *ho
     if (rc) return rc;
     rc = ln->din.ensure(pl.in_bytes);
     if (rc) return rc;
-    if (host_in) {
+    if (host_in && pl.C == 1) {
+        // one chunk at offset 0: straight from the caller's memory (the runtime
+        // stages pageable memory itself; a copy through our pinned buffer first
+        // costs a host pass over the chunk)
+        if (pl.in_end) HIPCHK(hipMemcpyAsync(ln->din.p, host_in, pl.in_end, hipMemcpyHostToDevice, ln->st));
+    } else if (host_in) {
         rc = ln->hin.ensure(pl.in_bytes);
         if (rc) return rc;
         uint8_t *h = (uint8_t *)ln->hin.p;
@@ -541,12 +546,10 @@ int lane_fetch_chunk(Lane *ln, uint32_t c, uint8_t *out, size_t out_cap, size_t 
     HIPCHK(hipStreamSynchronize(ln->st));
     *out_len = meta[1];
     if (meta[1] > out_cap) return fail(GPZ_E_NOSPACE, "output needs %llu bytes", (unsigned long long)meta[1]);
-    int rc = ln->hout.ensure(meta[1] + 8);
-    if (rc) return rc;
     if (meta[1]) {
-        HIPCHK(hipMemcpyAsync(ln->hout.p, W.work.out + meta[0], meta[1], hipMemcpyDeviceToHost, ln->st));
+        // straight into the caller's buffer (the runtime stages pageable memory)
+        HIPCHK(hipMemcpyAsync(out, W.work.out + meta[0], meta[1], hipMemcpyDeviceToHost, ln->st));
         HIPCHK(hipStreamSynchronize(ln->st));
-        memcpy(out, ln->hout.p, meta[1]);
     }
     return GPZ_OK;
 }
