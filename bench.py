@@ -326,12 +326,13 @@ def kernel_source_hash() -> str:
         return hashlib.sha256(f.read()).hexdigest()[:16]
 
 
-def profiled_counters(n_in: int) -> dict:
+def profiled_counters(n_in: int, k1_grid: int = 0) -> dict:
     """k_match_find counter figures as last measured with rocprofv3 --pmc (the
     counters cannot be read from inside this process): HBM bytes per launch
     (FETCH_SIZE and WRITE_SIZE passes, each scaled by a copy-kernel
-    calibration), VALU issue and lane-use fractions.  Empty if the committed
-    profile is for another input size or for other kernel sources (its
+    calibration) for the launch shape of k1_grid workgroups when the profile
+    has it, VALU issue and lane-use fractions.  Empty if the committed profile
+    is for another input size or for other kernel sources (its
     kernel_source_sha256 line differs from the current gpz_kernels.hip)."""
     path = os.path.join(ROOT, "profiles", "k_match_find_binmix256_traffic.md")
     vals = {}
@@ -339,7 +340,7 @@ def profiled_counters(n_in: int) -> dict:
         for line in open(path):
             k, _, v = line.partition(":")
             k = k.strip()
-            if k in ("traffic_bytes_per_launch", "input_bytes"):
+            if k in ("traffic_bytes_per_launch", "input_bytes") or k.startswith("traffic_bytes_per_launch_wg"):
                 vals[k] = int(v.strip())
             elif k in ("valu_issue_frac", "valu_lane_frac"):
                 vals[k] = float(v.strip())
@@ -349,6 +350,12 @@ def profiled_counters(n_in: int) -> dict:
         return {}
     if vals.get("input_bytes") != n_in or vals.get("kernel_source_sha256") != kernel_source_hash():
         return {}
+    shaped = vals.get(f"traffic_bytes_per_launch_wg{k1_grid}")
+    if shaped is not None:
+        vals["traffic_bytes_per_launch"] = shaped
+        vals["traffic_shape"] = f"{k1_grid} workgroups (the launch shape of this run)"
+    else:
+        vals["traffic_shape"] = "the shape the profile's own autotune picked"
     return vals
 
 
@@ -539,7 +546,9 @@ def main():
     alg_bytes = r["n_in"] + r["n_out"]  # per launch: sum over members of (n + c)
     k1_s = r["k1_ms"] / 1e3
     achieved = alg_bytes / k1_s / 1e9
-    pc = profiled_counters(r["n_in"])
+    tune = r["tune"] or {}
+    # the per-shape figures are of the two-workgroups-per-CU variant
+    pc = profiled_counters(r["n_in"], tune.get("k1_grid", 0) if tune.get("k1_variant", 1) in (0, 1) else 0)
     line = {
         "metric": "input GB/s on 1 MI355X (% HBM roofline) + ratio; p50/p99 request latency",
         "value": round(r["plain"]["gbps"], 3),
@@ -566,7 +575,7 @@ def main():
                      "traffic": pc.get("traffic_bytes_per_launch"), "alg_bytes_per_launch": alg_bytes,
                      "valu_issue_frac": pc.get("valu_issue_frac"), "valu_lane_frac": pc.get("valu_lane_frac"),
                      "traffic_source": ("profiles/k_match_find_binmix256_traffic.md (rocprofv3 --pmc, "
-                                        "same kernel sources)") if pc else
+                                        "same kernel sources), at " + pc["traffic_shape"]) if pc else
                                        "null: the committed counter profile is of other kernel sources",
                      "avg_launch_ms": round(r["k1_ms"], 4), "seg_table_ms": round(r["k0_ms"], 4),
                      "emit_ms": round(r["emit_ms"], 4),
