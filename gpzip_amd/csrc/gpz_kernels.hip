@@ -1400,7 +1400,6 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
     __shared__ HuffScratch hs;
     const uint32_t M = uni(w.counts->nmem);
     const uint32_t lane = threadIdx.x;
-    const uint64_t lt = (1ull << lane) - 1;
     for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
         const MemberDesc md = load_member(w.mem, m);
         for (uint32_t i = lane; i < NSYM; i += 64) freq[i] = w.hist[(uint64_t)m * NSYM + i];
