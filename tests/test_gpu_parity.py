@@ -469,6 +469,25 @@ def test_compress_chunk_with_1mib_members():
         c.close()
 
 
+@pytest.mark.parametrize("sub", [100003, 4000, 64])
+def test_member_sizes_off_the_segment_grid(sub):
+    """Members of sizes that are not multiples of the 4 KiB segment (a
+    partial last segment in every member; members shorter than a segment;
+    64-byte members): plain and batched forms (the batched one building its
+    tables on the device) equal the oracle's split at the same size."""
+    c = gpzip_amd.Context(sub_chunk=sub)
+    try:
+        chunks = [gpzip_amd.synth(k, s, 80 + i)[:s] for i, (k, s) in
+                  enumerate([("text", 300001), ("binmix", 4096), ("log", 70000), ("text", 1), ("binmix", 0)])]
+        say("members of", sub, "bytes")
+        ref = [O.gzip_chunk(x, sub) for x in chunks]
+        for x, r in zip(chunks, ref):
+            assert c.compress_chunk(x) == r
+        assert c.batch(chunks) == ref
+    finally:
+        c.close()
+
+
 def _oracle_many(chunks, sub=131072):
     import concurrent.futures as cf
 
