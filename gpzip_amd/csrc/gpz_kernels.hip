@@ -720,14 +720,25 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
             // hash does not occur there (the reference's empty slot)
             uint32_t fv[4], pd[4] = {0, 0, 0, 0};
             {
-                const uint16_t *rk = reinterpret_cast<const uint16_t *>(tab + TAB_RANK);
+                const uint8_t *tb = reinterpret_cast<const uint8_t *>(tab);
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const uint32_t h = ((i < 2 ? hh0 : hh1) >> (16 * (i & 1))) & 0xFFFFu;
-                    const uint32_t wd = tab[TAB_BITS + (h >> 5)];
-                    const uint32_t below = wd & ((1u << (h & 31u)) - 1u);
-                    const uint32_t e = tab[TAB_ENT + rk[h >> 5] + __popc(below)];
-                    fv[i] = (((hvm >> i) & 1u) && ((wd >> (h & 31u)) & 1u) && (st[i] >> 16) < MAX_MATCH) ? e : EMPTY;
+                    // hash h = bits 16 (i & 1) .. +15 of the packed pair: bitmap
+                    // word h >> 5; the bit-field builtins take h & 31 from the
+                    // low five bits of `hb`
+                    const uint32_t hw = i < 2 ? hh0 : hh1;
+                    const uint32_t wi = __builtin_amdgcn_ubfe(hw, 16 * (i & 1) + 5, 11);
+                    const uint32_t hb = (i & 1) ? hw >> 16 : hw;
+                    uint32_t e = EMPTY;
+                    if (((hvm >> i) & 1u) && (st[i] >> 16) < MAX_MATCH) {
+                        const uint32_t wd = tab[TAB_BITS + wi];
+                        if (__builtin_amdgcn_ubfe(wd, hb, 1)) {
+                            const uint32_t r = *reinterpret_cast<const uint16_t *>(tb + 4 * TAB_RANK + 2 * wi);
+                            e = *reinterpret_cast<const uint32_t *>(
+                                tb + 4 * TAB_ENT + 4 * (r + __popc(__builtin_amdgcn_ubfe(wd, 0, hb))));
+                        }
+                    }
+                    fv[i] = e;
                 }
             }
             // every wave has its entries: tab is free for the next table
@@ -749,8 +760,8 @@ __global__ __launch_bounds__(K1_T, MINB) void k_match_find(Work w, int do_summar
                 if (v != EMPTY) {
                     const uint32_t p = S + xk + i;
                     const uint32_t lim = FAST ? MAX_MATCH : min(n - p, MAX_MATCH);
-                    const uint32_t own0 = __builtin_amdgcn_alignbyte(own_hi, own_lo, i);
-                    const uint32_t own1 = __builtin_amdgcn_alignbyte(own_w2, own_hi, i);
+                    const uint32_t own0 = i ? __builtin_amdgcn_alignbyte(own_hi, own_lo, i) : own_lo;
+                    const uint32_t own1 = i ? __builtin_amdgcn_alignbyte(own_w2, own_hi, i) : own_hi;
                     const uint32_t cn = base + (v & 0xFFFu), co = base + ((v >> 12) & 0xFFFu);
                     const uint32_t ln = cand_short<FAST>(win, WB, lim, p, cn, w.window, own0, own1);
                     if (ln & PEND) {
