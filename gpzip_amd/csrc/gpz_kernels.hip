@@ -37,7 +37,7 @@ namespace gpz {
 // ---------------------------------------------------------------------------
 // constants
 
-__constant__ uint32_t c_crc_tab[256];   // reflected CRC-32 byte table
+__constant__ uint32_t c_crc_tab[1024];  // reflected CRC-32 slicing-by-4 tables T0 .. T3
 __constant__ uint32_t c_xpowb[4096];    // x^(8*b) mod P, reflected, b < 4096 (bytes within a segment)
 __constant__ uint32_t c_segpow[4097];   // x^(8*4096*j) mod P, reflected (segments of 4 KiB)
 
@@ -358,18 +358,21 @@ __global__ __launch_bounds__(BT) void k_seg_table(Work w) {
     __shared__ uint16_t rk[2048];
     __shared__ uint32_t fst[SEG];
     __shared__ uint32_t lst[SEG];
-    __shared__ uint32_t crc_tab[256];
+    __shared__ uint32_t crc_tab[4][256];
     __shared__ uint32_t tmp[16];
     const uint32_t tid = threadIdx.x;
     const uint32_t G = uni(w.counts->nseg);
-    for (uint32_t i = tid; i < 256; i += BT) crc_tab[i] = c_crc_tab[i];
+    for (uint32_t i = tid; i < 1024; i += BT) crc_tab[i >> 8][i & 255] = c_crc_tab[i];
     // bytes x0 .. x0 + 4*NW - 1 of a segment (buffers are padded past every
     // member), loaded one segment ahead so the loads are in flight while the
-    // current one is processed
+    // current one is processed; lanes of wave 0 also load the 64 bytes of
+    // their CRC run (64 lane .. 64 lane + 63)
     const uint32_t x0 = tid * PPT;
-    auto fetch = [&](uint32_t gg, uint32_t (&d)[NW]) {
+    auto fetch = [&](uint32_t gg, uint32_t (&d)[NW], uint4 (&cq)[4]) {
 #pragma unroll
         for (uint32_t j = 0; j < NW; j++) d[j] = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) cq[j] = make_uint4(0, 0, 0, 0);
         if (gg >= G) return;
         const MemberDesc mm = load_member(w.mem, sload_u32(w.seg_mem, gg));
         const uint32_t SS = (gg - mm.seg_base) << SEG_LOG2, LL = min(SS + SEG, mm.n) - SS;
@@ -378,9 +381,17 @@ __global__ __launch_bounds__(BT) void k_seg_table(Work w) {
 #pragma unroll
             for (uint32_t j = 0; j < NW; j++) d[j] = src[j];
         }
+        if (tid < 64 && 64u * tid < LL) {
+            // member starts are 16-byte aligned, and at most 63 bytes past a
+            // member's end are read (the input's 256-byte read slack)
+            const uint4 *src = reinterpret_cast<const uint4 *>(w.in + mm.in_off + SS + 64u * tid);
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) cq[j] = src[j];
+        }
     };
     uint32_t nx[NW];
-    fetch(xcd_block(), nx);
+    uint4 ncq[4];
+    fetch(xcd_block(), nx, ncq);
     for (uint32_t g = xcd_block(); g < G; g += gridDim.x) {
         const MemberDesc md = load_member(w.mem, sload_u32(w.seg_mem, g));
         const uint32_t n = md.n, s = g - md.seg_base;
@@ -389,20 +400,38 @@ __global__ __launch_bounds__(BT) void k_seg_table(Work w) {
         uint32_t d[NW];
 #pragma unroll
         for (uint32_t j = 0; j < NW; j++) d[j] = nx[j];
+        uint32_t cw[16];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) {
+            cw[4 * j] = ncq[j].x;
+            cw[4 * j + 1] = ncq[j].y;
+            cw[4 * j + 2] = ncq[j].z;
+            cw[4 * j + 3] = ncq[j].w;
+        }
         auto byte_at = [&](uint32_t i) { return (d[i >> 2] >> (8 * (i & 3))) & 0xFFu; };
 #pragma unroll
         for (uint32_t r = 0; r < WPT; r++) bm[tid + r * BT] = 0;
         __syncthreads();
-        // raw CRC-32 of the segment, started here so that its chain of table
-        // lookups overlaps the atomics and barriers below: PPT bytes per
-        // thread, folded by x^(8 * bytes after them), reduced at the end
+        // raw CRC-32 of the segment on wave 0 alone (the other waves skip its
+        // instructions): lane l takes bytes 64 l .. 64 l + 63, four at a time
+        // through the slicing-by-4 tables (Kounavis and Berry), then folds its
+        // CRC by x^(8 * bytes after them); reduced over the wave at the end
         uint32_t crc = 0;
-        {
+        if (tid < 64 && 64u * tid < L) {
+            const uint32_t cnt = min(64u, L - 64u * tid);
 #pragma unroll
-            for (uint32_t b = 0; b < PPT; b++) {
-                if (x0 + b < L) crc = crc_tab[(crc ^ byte_at(b)) & 0xFF] ^ (crc >> 8);
+            for (uint32_t j = 0; j < 16; j++) {
+                if (4 * j + 4 <= cnt) {
+                    const uint32_t c = crc ^ cw[j];
+                    crc = crc_tab[3][c & 0xFF] ^ crc_tab[2][(c >> 8) & 0xFF] ^ crc_tab[1][(c >> 16) & 0xFF] ^
+                          crc_tab[0][c >> 24];
+                } else {
+#pragma unroll
+                    for (uint32_t b = 0; b < 3; b++)
+                        if (4 * j + b < cnt) crc = crc_tab[0][(crc ^ (cw[j] >> (8 * b))) & 0xFF] ^ (crc >> 8);
+                }
             }
-            crc = x0 < L ? gf2_mul(crc, c_xpowb[L - min(x0 + PPT, L)]) : 0;
+            crc = gf2_mul(crc, c_xpowb[L - 64u * tid - cnt]);
         }
         uint32_t hh[PPT];
         uint32_t hvm = 0;
@@ -417,7 +446,7 @@ __global__ __launch_bounds__(BT) void k_seg_table(Work w) {
         // the bytes are in registers now: the next segment's loads go out
         // without a wait for them
         asm volatile("" ::: "memory");
-        fetch(g + gridDim.x, nx);
+        fetch(g + gridDim.x, nx, ncq);
         __syncthreads();
         uint32_t D;  // distinct hashes in the segment
         {
@@ -455,17 +484,11 @@ __global__ __launch_bounds__(BT) void k_seg_table(Work w) {
         }
         __syncthreads();
         for (uint32_t id = tid; id < D; id += BT) T[TAB_ENT + id] = (fst[id] << 12) | lst[id];
-        {
+        if (tid < 64) {
             uint32_t c = crc;
 #pragma unroll
             for (int dd = 32; dd >= 1; dd >>= 1) c ^= __shfl_xor(c, dd, 64);
-            if ((tid & 63) == 0) tmp[tid >> 6] = c;
-            __syncthreads();
-            if (tid == 0) {
-                uint32_t t = 0;
-                for (int i = 0; i < BT / 64; i++) t ^= tmp[i];
-                w.seg_crc[g] = t;
-            }
+            if (tid == 0) w.seg_crc[g] = c;
         }
         __syncthreads();
     }
@@ -1924,7 +1947,15 @@ hipError_t upload_constants() {
     const uint32_t seg1 = mul(xpowb[4095], x8);  // x^(8 * 4096)
     segpow[0] = 0x80000000u;
     for (uint32_t j = 1; j <= 4096; j++) segpow[j] = mul(segpow[j - 1], seg1);
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_tab), tab, sizeof(tab));
+    // slicing-by-4: T_k[i] is the CRC of byte i followed by k zero bytes
+    uint32_t tab4[1024];
+    for (uint32_t i = 0; i < 256; i++) tab4[i] = tab[i];
+    for (uint32_t k = 1; k < 4; k++)
+        for (uint32_t i = 0; i < 256; i++) {
+            const uint32_t p = tab4[(k - 1) * 256 + i];
+            tab4[k * 256 + i] = (p >> 8) ^ tab[p & 0xFF];
+        }
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_tab), tab4, sizeof(tab4));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_segpow), segpow, sizeof(segpow));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_xpowb), xpowb, sizeof(xpowb));
     return e;
