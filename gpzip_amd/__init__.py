@@ -53,7 +53,8 @@ class Desc(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("k0_grid", C.c_uint32), ("k1_grid", C.c_uint32), ("k3_grid", C.c_uint32), ("emit_grid", C.c_uint32),
                 ("k1_variant", C.c_uint32), ("emit_variant", C.c_uint32), ("k0_variant", C.c_uint32),
-                ("k1_seg_per_wg", C.c_uint32), ("k3_seg_per_wg", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("k1_seg_per_wg", C.c_uint32), ("k3_seg_per_wg", C.c_uint32), ("walk_variant", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

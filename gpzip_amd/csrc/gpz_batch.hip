@@ -295,9 +295,9 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
         p->grids.emit = p->grids.huff = p->grids.chain = std::max<uint32_t>(1, p->pl.M);
     }
     if (tune) {
-        if (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3) {
+        if (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3 || tune->walk_variant > 2) {
             delete p;
-            return fail(GPZ_E_INVAL, "k1_variant and emit_variant must be 0-2, k0_variant 0-3");
+            return fail(GPZ_E_INVAL, "k1_variant, emit_variant and walk_variant must be 0-2, k0_variant 0-3");
         }
         apply_tune(p->grids, *tune, p->pl.G, p->pl.M);
     } else if (flags & GPZ_PLAN_PERSISTENT) {
@@ -353,7 +353,21 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
     // segments per workgroup -> grid on this batch
     auto spw = [&](uint32_t n) { return (G + n - 1) / n; };
     uint32_t best_k1_spw = 0, best_k3_spw = 0;
-    {  // match-find: occupancy variant x segments per workgroup (1 = one workgroup per segment)
+    {  // walk: per segment (summaries in match-find, chain, k_walk_emit) or per member;
+       // the interval holds match-find, the walk, Huffman and scan
+        float bt = 1e30f;
+        for (uint32_t v = 0; v < 2 && !rc; v++) {
+            Grids g = best;
+            g.walkv = v;
+            float t;
+            if ((rc = timed(g, 5, 3, &t))) break;
+            if (t < bt) {
+                bt = t;
+                best.walkv = v;
+            }
+        }
+    }
+    if (!rc) {  // match-find: occupancy variant x segments per workgroup (1 = one workgroup per segment)
         const uint32_t spws[] = {1, 2, 4, 8, 16, 32, 64};
         float bt = 1e30f;
         for (uint32_t v = 0; v < 2 && !rc; v++) {
@@ -391,7 +405,7 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
             }
         }
     }
-    if (!rc) {  // walk / token emission (with chain, Huffman and scan in the interval): segments per workgroup
+    if (!rc && !best.walkv) {  // per-segment walk / token emission (with chain, Huffman and scan in the interval): segments per workgroup
         const uint32_t spws[] = {1, 2, 4, 8, 16, 32};
         float bt = 1e30f;
         for (uint32_t n : spws) {
@@ -438,6 +452,7 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
     out->k0_variant = best.k0v == 0 ? 2 : (best.k0v == 1 ? 1 : 3);
     out->k1_seg_per_wg = best_k1_spw;
     out->k3_seg_per_wg = best_k3_spw;
+    out->walk_variant = best.walkv + 1;
     if (ms) {
         ms[0] = t1;
         ms[1] = t0;

@@ -1147,6 +1147,200 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_walk_member: one wave walks a whole member in position order, 64
+// positions (one block) at a time, from position 0 -- the reference's walk
+// (lz77.rs:260-284; strict rule walk.rs:512-525; rule 2 the opt-in ratio
+// mode).  Per block the lanes compute their successors; the orbit through the
+// block is then followed on the scalar unit, one step per match taken: the
+// next position whose successor is not the next position is the first set
+// bit of a ballot, its successor one readlane.  The walked set is rebuilt on
+// the vector unit from the matches taken (a position is walked when it is at
+// or past the entry and at or past the end of the last match taken before
+// it: a wave max-scan).  The walked tokens are compacted into the member's
+// token stream and counted into a wave-private histogram that is stored
+// whole at the end (no global atomics, no clearing pass).  Without
+// per-segment walk summaries there is no chaining pass either.  Tokens and
+// input bytes are loaded one group of KM_G blocks ahead.  This is the walk of
+// large batches: a member's blocks run in sequence, so a batch needs many
+// members to fill the chip.
+
+constexpr int KM_T = 256;     // four waves per workgroup, each its own member
+constexpr uint32_t KM_G = 8;  // blocks of 64 positions per group
+constexpr uint32_t HIST_USED = 316;  // litlen 0..285, distance 286..315; 319 takes invalid tokens
+
+// One group of KM_G blocks in registers: lane l's token of each block, of the
+// first block of the next group (the walk rule reads ahead), and 8 input
+// bytes (the group's 512 bytes spread 8 per lane, for demoted matches).
+struct WalkGroup {
+    uint32_t t[KM_G + 1];
+    uint2 r;
+};
+
+// Lane i receives lane i + 1's v, lane 63 receives `last` (DPP wave_shl:1).
+__device__ __forceinline__ uint32_t lane_next(uint32_t v, uint32_t last) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)v, 0x130, 0xf, 0xf, false);
+}
+
+// Maximum of v over the lanes below this one (0 on lane 0): DPP row shifts and
+// row broadcasts as in wave_scan_incl, then wave_shr:1.
+__device__ __forceinline__ uint32_t wave_max_excl(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));  // row_shr:1
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));  // row_shr:2
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));  // row_shr:4
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));  // row_shr:8
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);        // wave_shr:1
+}
+
+// Histogram slots of a walked token, without branches: the literal or length
+// symbol (HIST_USED + 3 for a token outside the alphabet) and, for a match,
+// the distance symbol.
+__device__ __forceinline__ uint32_t tok_bin1(uint32_t to, bool &match) {
+    const uint32_t tl = to >> 16, tv = to & 0xFFFFu;
+    const bool okl = tl == 0 && tv < 256;
+    match = tl >= 3 && tl <= 258 && tv >= 1 && tv <= 32768;
+    const uint32_t l = tl - 3, kl = 31 - __clz(max(l, 8u));
+    const uint32_t li = tl <= 10 ? l : (tl == 258 ? 28u : 4 * (kl - 1) + ((l >> (kl - 2)) & 3u));
+    return okl ? tv : (match ? 257 + li : HIST_USED + 3);
+}
+__device__ __forceinline__ uint32_t tok_bin2(uint32_t to) {
+    const uint32_t d = to & 0xFFFFu, x = max(d, 5u) - 1, k = 31 - __clz(x);
+    return DSYM0 + (d <= 4 ? d - 1 : 2 * k + ((x >> (k - 1)) & 1u));
+}
+
+template <int RULE>
+__device__ __forceinline__ void walk_member(const Work &w, uint32_t m, uint32_t *hist, uint32_t *stg) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const MemberDesc md = load_member(w.mem, m);
+    const uint32_t n = md.n;
+    if (n == 0) {  // an empty chunk's member: no tokens
+        for (uint32_t i = lane; i < NSYM; i += 64) w.hist[(uint64_t)m * NSYM + i] = 0;
+        if (lane == 0) w.ntok[m] = 0;
+        return;
+    }
+    const uint32_t *tk = w.tok + md.pos_base;
+    const uint8_t *ib = w.in + md.in_off;
+    uint32_t *outp = w.walked + md.pos_base;
+    for (uint32_t i = lane; i < NSYM; i += 64) hist[i] = 0;
+    wave_lds_sync();
+    const uint32_t nb = (n + 63u) >> 6, ng = (nb + KM_G - 1) / KM_G;
+    // Loads are unconditional, their addresses clamped into the member (token
+    // at the last position, input bytes at the last 8-byte boundary); what
+    // lies past the member end is zeroed or ignored where it is used.
+    const uint32_t xl = n - 1, rl = (n - 1) & ~7u;
+    auto load = [&](WalkGroup &G, uint32_t gi) {
+        const uint32_t x0 = gi * (64u * KM_G) + lane;
+#pragma unroll
+        for (uint32_t k = 0; k <= KM_G; k++) G.t[k] = tk[min(x0 + 64u * k, xl)];
+        if (RULE != 1)
+            G.r = *reinterpret_cast<const uint2 *>(ib + min(gi * (64u * KM_G) + 8u * lane, rl));
+    };
+    // Loads and stores share one counter on this target, so the compiler's
+    // waits before the use of a loaded register drain it; there is one
+    // explicit drain per group instead, with the group's loads issued a whole
+    // group earlier and the previous group's stores issued just as early:
+    // walked tokens are staged in LDS (stg, 64 * KM_G words per wave) and
+    // stored at the start of the next group.
+    uint32_t e = 0;      // next walked position (member-relative)
+    uint32_t outc = 0;   // tokens stored so far
+    uint32_t scnt = 0;   // tokens staged
+    auto flush = [&]() {
+        wave_lds_sync();
+        for (uint32_t i = lane; i < scnt; i += 64) outp[outc + i] = stg[i];
+        outc += scnt;
+        scnt = 0;
+        wave_lds_sync();  // the reads are done before the next group's writes
+    };
+    WalkGroup A, B;  // A: the group being walked (loaded), B: the next (in flight)
+    load(B, 0);
+    for (uint32_t gi = 0; gi < ng; gi++) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): group gi and the stores before it
+        A = B;
+        if (gi + 1 < ng) load(B, gi + 1);
+        flush();
+#pragma unroll
+        for (uint32_t k = 0; k < KM_G; k++) {
+            const uint32_t bs = (gi * KM_G + k) << 6;
+            if (bs >= n) break;
+            if (e >= bs + 64u) continue;  // (uniform) a match jumps over the block
+            const bool in = bs + lane < n;
+            const uint64_t valid = __ballot(in);
+            const uint32_t t = in ? A.t[k] : 0u, len = t >> 16;
+            // tokens at x + 1 (and x + 2): the next lane, past lane 63 the next
+            // block's first lanes; 0 past the member end
+            const uint32_t rn0 = (uint32_t)__builtin_amdgcn_readlane((int)A.t[k + 1], 0);
+            const uint32_t tn0 = bs + 64u < n ? rn0 : 0u;
+            const uint32_t t1 = lane_next(t, tn0);
+            bool dem = false;
+            if (RULE == 0) {
+                dem = len != 0 && (t1 >> 16) > len;
+            } else if (RULE == 2) {
+                const uint32_t rn1 = (uint32_t)__builtin_amdgcn_readlane((int)A.t[k + 1], 1);
+                const uint32_t t2 = lane_next(t1, bs + 65u < n ? rn1 : 0u);
+                dem = len != 0 && ((t1 >> 16) > len || (t2 >> 16) > len + 1);
+            }
+            const uint32_t s = (len == 0 || dem) ? 1u : len;  // successor distance
+            // where a run of next-position successors stops: a match taken, or
+            // lane 63 (its successor leaves the block either way)
+            const uint64_t stop = ~__ballot(s == 1) | (1ull << 63);
+            const uint32_t c0 = e - bs;
+            uint32_t cur = c0;
+            uint64_t jm = 0;  // positions where the walk takes a match
+            do {
+                const uint32_t mm = cur + (uint32_t)__builtin_ctzll(stop >> cur);
+                jm |= 1ull << mm;
+                cur = mm + (uint32_t)__builtin_amdgcn_readlane((int)s, (int)mm);
+            } while (cur < 64u);
+            e = bs + cur;
+            uint64_t V = valid & (~0ull << c0);
+            if (jm & (~0ull >> 1)) {
+                // past a match taken at m the walk resumes at m + s(m)
+                const uint32_t r = wave_max_excl(((jm >> lane) & 1ull) ? lane + s : 0u);
+                V &= __ballot(lane >= r);
+            }
+            uint32_t to = t;
+            if (RULE != 1 && (__ballot(dem) & V)) {
+                // a demoted match is its literal byte: position 64 k + lane of
+                // the group is byte lane & 7 of lane 8 k + lane / 8
+                const int src = (int)(8u * k + (lane >> 3));
+                const uint32_t lo = (uint32_t)__shfl((int)A.r.x, src, 64);
+                const uint32_t hi = (uint32_t)__shfl((int)A.r.y, src, 64);
+                const uint32_t rb = (((lane & 4u) ? hi : lo) >> (8u * (lane & 3u))) & 0xFFu;
+                if (dem) to = rb;
+            }
+            bool match;
+            const uint32_t b1 = tok_bin1(to, match);
+            const uint32_t b2 = tok_bin2(to);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(V >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)V, 0u));
+            if ((V >> lane) & 1ull) {
+                stg[scnt + rank] = to;
+                atomicAdd(&hist[b1], 1u);
+                if (match) atomicAdd(&hist[b2], 1u);
+            }
+            scnt += (uint32_t)__popcll(V);
+        }
+    }
+    flush();
+    for (uint32_t i = lane; i < NSYM; i += 64) w.hist[(uint64_t)m * NSYM + i] = i < HIST_USED ? hist[i] : 0u;
+    if (lane == 0) w.ntok[m] = outc;
+    wave_lds_sync();  // the histogram is cleared again for the next member
+}
+
+__global__ __launch_bounds__(KM_T) void k_walk_member(Work w) {
+    __shared__ uint32_t hs[KM_T / 64][NSYM];
+    __shared__ uint32_t stg[KM_T / 64][64 * KM_G];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t M = uni(w.counts->nmem);
+    const uint32_t rule = uni(w.strict);
+    for (uint32_t m = uni(blockIdx.x * (KM_T / 64) + wv); m < M; m += gridDim.x * (KM_T / 64)) {
+        if (rule == 0) walk_member<0>(w, m, hs[wv], stg[wv]);
+        else if (rule == 1) walk_member<1>(w, m, hs[wv], stg[wv]);
+        else walk_member<2>(w, m, hs[wv], stg[wv]);
+    }
+}
+
 // Histogram of caller-supplied walked tokens (stage-level gpz_encode_block).
 __global__ __launch_bounds__(1024) void k_hist_tokens(Work w) {
     __shared__ uint32_t hist[NSYM];
@@ -1983,6 +2177,9 @@ void launch_chain(const Work &w, uint32_t grid, hipStream_t st) {
 }
 void launch_walk_emit(const Work &w, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_walk_emit, dim3(grid), dim3(K3_T), 0, st, w);
+}
+void launch_walk_member(const Work &w, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_walk_member, dim3(grid), dim3(KM_T), 0, st, w);
 }
 void launch_hist_tokens(const Work &w, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_hist_tokens, dim3(grid), dim3(1024), 0, st, w);

@@ -33,6 +33,7 @@ void launch_match_find(const Work &w, uint32_t grid, int summary, int variant, h
 void launch_walk_summary(const Work &w, uint32_t grid, hipStream_t st);
 void launch_chain(const Work &w, uint32_t grid, hipStream_t st);
 void launch_walk_emit(const Work &w, uint32_t grid, hipStream_t st);
+void launch_walk_member(const Work &w, uint32_t grid, hipStream_t st);
 void launch_hist_tokens(const Work &w, uint32_t grid, hipStream_t st);
 void launch_huffman(const Work &w, uint32_t grid, hipStream_t st);
 void launch_scan(const Work &w, hipStream_t st);
@@ -333,6 +334,10 @@ Grids grids_for(const Caps &c, int n_cu) {
     g.chain = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 16));
     g.huff = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 16));
     g.emit = (uint32_t)std::max<uint64_t>(1, c.M);
+    // the per-member walk runs a member's blocks in sequence: it pays once a
+    // batch has enough members to fill the chip, the per-segment walk below that
+    g.km = (uint32_t)std::max<uint64_t>(1, (c.M + 3) / 4);
+    g.walkv = c.M >= 512 ? 1 : 0;
     return g;
 }
 
@@ -347,6 +352,7 @@ void apply_tune(Grids &g, const gpz_tune &t, uint64_t G, uint64_t M) {
     if (t.k1_variant == 1 || t.k1_variant == 2) g.k1v = t.k1_variant - 1;
     if (t.emit_variant == 1 || t.emit_variant == 2) g.emitv = t.emit_variant - 1;
     if (t.k0_variant >= 1 && t.k0_variant <= 3) g.k0v = t.k0_variant == 2 ? 0 : (t.k0_variant == 1 ? 1 : 2);
+    if (t.walk_variant == 1 || t.walk_variant == 2) g.walkv = t.walk_variant - 1;
     // a record tuned on a larger batch must not launch workgroups with no
     // segment or member to take
     auto cap = [](uint32_t v, uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(v, n)); };
@@ -366,9 +372,13 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
             launch_match_find(w, g.k1, 0, g.k1v, st);
             break;
         case MODE_WALK:
-            launch_walk_summary(w, g.k1, st);
-            launch_chain(w, g.chain, st);
-            launch_walk_emit(w, g.k3, st);
+            if (g.walkv) {
+                launch_walk_member(w, g.km, st);
+            } else {
+                launch_walk_summary(w, g.k1, st);
+                launch_chain(w, g.chain, st);
+                launch_walk_emit(w, g.k3, st);
+            }
             break;
         case MODE_ENCODE:
             launch_hist_tokens(w, g.huff, st);
@@ -382,18 +392,25 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
             if (w.cdesc) launch_plan_batch(w, st);  // tables from the chunk descriptors
             launch_seg_table(w, g.k0, g.k0v, st);
             if (ev) hipEventRecord(ev[5], st);
-            if (w.strict == 2) {
-                // ratio mode: the walk rule reads two tokens ahead, so the
-                // summary runs once every token exists
+            if (g.walkv) {
+                // per-member walk: no summaries, no chaining pass
                 launch_match_find(w, g.k1, 0, g.k1v, st);
                 if (ev) hipEventRecord(ev[1], st);
-                launch_walk_summary(w, g.k1, st);
+                launch_walk_member(w, g.km, st);
             } else {
-                launch_match_find(w, g.k1, 1, g.k1v, st);  // with the walk summary
-                if (ev) hipEventRecord(ev[1], st);
+                if (w.strict == 2) {
+                    // ratio mode: the walk rule reads two tokens ahead, so the
+                    // summary runs once every token exists
+                    launch_match_find(w, g.k1, 0, g.k1v, st);
+                    if (ev) hipEventRecord(ev[1], st);
+                    launch_walk_summary(w, g.k1, st);
+                } else {
+                    launch_match_find(w, g.k1, 1, g.k1v, st);  // with the walk summary
+                    if (ev) hipEventRecord(ev[1], st);
+                }
+                launch_chain(w, g.chain, st);
+                launch_walk_emit(w, g.k3, st);
             }
-            launch_chain(w, g.chain, st);
-            launch_walk_emit(w, g.k3, st);
             launch_huffman(w, g.huff, st);
             launch_scan(w, st);
             if (ev) hipEventRecord(ev[3], st);
@@ -496,6 +513,16 @@ int ctx_cus(gpz_ctx *c) { return c->n_cu; }
 uint32_t ctx_sub_chunk(gpz_ctx *c) { return c->cfg.sub_chunk; }
 uint32_t ctx_walk_rule(gpz_ctx *c) { return (c->cfg.flags & GPZ_CFG_RATIO) ? 2u : 0u; }
 const gpz_tune &ctx_tune(gpz_ctx *c) { return c->tune; }
+
+// Grids of the per-lane launches (single chunks, stage-level calls): the
+// built-in shapes, with the walk of the context's tuning record when it names
+// one.
+static Grids lane_grids(gpz_ctx *ctx, const Caps &c) {
+    Grids g = grids_for(c, ctx->n_cu);
+    const uint32_t wv = ctx->tune.walk_variant;
+    if (wv == 1 || wv == 2) g.walkv = wv - 1;
+    return g;
+}
 Lane *ctx_acquire(gpz_ctx *c) { return c->acquire(); }
 void ctx_release(gpz_ctx *c, Lane *l) { c->release(l); }
 
@@ -531,7 +558,7 @@ int lane_run(gpz_ctx *ctx, Lane *ln, const Plan &pl, int mode, const uint8_t *ho
     w.in = (const uint8_t *)ln->din.p;
     w.gzip = pl.gzip;
     w.strict = ln->strict;
-    Grids g = grids_for(ln->wsp.caps, ctx->n_cu);
+    Grids g = lane_grids(ctx, ln->wsp.caps);
     enqueue(w, g, mode, ln->st, nullptr);
     HIPCHK(hipGetLastError());
     return GPZ_OK;
@@ -564,8 +591,8 @@ extern "C" {
 int gpz_set_tune(gpz_ctx *ctx, const gpz_tune *tune) {
     clear_error();
     if (!ctx) return fail(GPZ_E_INVAL, "null context");
-    if (tune && (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3))
-        return fail(GPZ_E_INVAL, "k1_variant and emit_variant must be 0-2, k0_variant 0-3");
+    if (tune && (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3 || tune->walk_variant > 2))
+        return fail(GPZ_E_INVAL, "k1_variant, emit_variant and walk_variant must be 0-2, k0_variant 0-3");
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (tune) ctx->tune = *tune;
     else ctx->tune = gpz_tune{};
@@ -742,10 +769,11 @@ int gpz_compress_chunk_noalloc(gpz_ctx *ctx, const uint8_t *in, size_t len, uint
     w.gzip = 1;
     w.strict = ctx_walk_rule(ctx);
     // no array needs to start at zero: every kernel writes what it later
-    // reads (k_chain clears the histograms), as on reused lane workspaces
+    // reads (the per-member walk stores the histograms whole, k_chain clears
+    // them for the per-segment walk), as on reused lane workspaces
     if (len) HIPCHK(hipMemcpyAsync(base + in_at, in, len, hipMemcpyDefault, st));
     launch_plan_one(w, (uint32_t)len, (uint32_t)sub, st);
-    enqueue(w, grids_for(c, ctx->n_cu), MODE_FULL, st, nullptr);
+    enqueue(w, lane_grids(ctx, c), MODE_FULL, st, nullptr);
     HIPCHK(hipGetLastError());
     uint64_t meta[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(&meta[0], w.chunk_off, 8, hipMemcpyDeviceToHost, st));
@@ -851,7 +879,7 @@ int gpz_walk(gpz_ctx *ctx, const uint8_t *in, const uint32_t *per_pos, size_t le
         w.in = (const uint8_t *)ln->din.p;
         w.strict = ln->strict;
         w.gzip = 1;
-        enqueue(w, grids_for(ln->wsp.caps, ctx->n_cu), MODE_WALK, ln->st, nullptr);
+        enqueue(w, lane_grids(ctx, ln->wsp.caps), MODE_WALK, ln->st, nullptr);
         uint32_t nt = 0;
         if (hipMemcpyAsync(&nt, w.ntok, 4, hipMemcpyDeviceToHost, ln->st) != hipSuccess ||
             hipStreamSynchronize(ln->st) != hipSuccess) {
@@ -913,7 +941,7 @@ int gpz_encode_block(gpz_ctx *ctx, const uint32_t *walked, size_t n, uint8_t *ou
         if ((rc = ln->wsp.upload_plan(pl, ln->st))) break;
         w.in = (const uint8_t *)ln->din.p;
         w.gzip = 0;
-        enqueue(w, grids_for(ln->wsp.caps, ctx->n_cu), MODE_ENCODE, ln->st, nullptr);
+        enqueue(w, lane_grids(ctx, ln->wsp.caps), MODE_ENCODE, ln->st, nullptr);
         rc = lane_fetch_chunk(ln, 0, out, out_cap, out_len);
     } while (0);
     if (rc) hipStreamSynchronize(ln->st);

@@ -201,12 +201,17 @@ typedef struct gpz_tune {
                               2 = 512 (8), 3 = 1024 (4); 0 = built-in (2) */
     uint32_t k1_seg_per_wg; /* match-find segments per workgroup; 0 = use k1_grid */
     uint32_t k3_seg_per_wg; /* walk/emit-tokens segments per workgroup; 0 = use k3_grid */
-    uint32_t reserved[3];
+    uint32_t walk_variant;  /* walk: 1 = per segment (walk summaries in the match-find kernel,
+                               a chaining pass, one workgroup per segment), 2 = per member
+                               (one wave walks each member in order); 0 = built-in (2 for
+                               batches of 512 members or more, else 1) */
+    uint32_t reserved[2];
 } gpz_tune;
 
 /* Tuning record used by every batched launch of this context (gpz_batch_*,
  * the serving form, and gpz_plan_create with GPZ_PLAN_PERSISTENT and a NULL
- * tune).  NULL restores the built-in grids.  No reference counterpart: the
+ * tune); its walk_variant also applies to the single-chunk and stage-level
+ * calls.  NULL restores the built-in grids.  No reference counterpart: the
  * reference's batching constants are fixed (codec-gpu/src/gpu/batch.rs:31-35). */
 int gpz_set_tune(gpz_ctx *ctx, const gpz_tune *tune);
 int gpz_get_tune(gpz_ctx *ctx, gpz_tune *tune);

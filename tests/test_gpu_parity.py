@@ -95,6 +95,99 @@ def test_walk_equals_oracle(ctx, kind, n):
     assert_tokens_equal(ctx.walk(t, strict=True), O.strict_walk(t), f"strict {kind} {n}")
 
 
+# ---- the per-member walk (k_walk_member), forced on every call of a context ----
+
+@pytest.fixture(scope="module")
+def ctx_mw():
+    c = gpzip_amd.Context()
+    t = gpzip_amd.Tune()
+    t.walk_variant = 2
+    c.set_tune(t)
+    yield c
+    c.close()
+
+
+def test_member_walk_reference_cases(ctx_mw):
+    """The reference's strict-walk cases and the hand-derived lazy cases
+    through the per-member walk."""
+    for case in GOLD["walk_strict"]:
+        if "all_literals" in case:
+            toks = pack([(0, i & 0xFF) for i in range(case["all_literals"])])
+            exp = toks
+        else:
+            toks, exp = pack(case["tokens"]), pack(case["expect"])
+        say("member walk strict", case["name"], toks.size)
+        assert_tokens_equal(ctx_mw.walk(toks, strict=True), exp, case["name"])
+    for case in GOLD["walk_lazy"]:
+        data = base64.b64decode(case["data_b64"])
+        say("member walk lazy", case["name"])
+        assert_tokens_equal(ctx_mw.walk(pack(case["tokens"]), data), pack(case["expect"]), case["name"])
+
+
+@pytest.mark.parametrize("kind", ["text", "binmix", "log", "zeros"])
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 127, 511, 512, 513, 4097, 70000, 131072])
+def test_member_walk_equals_oracle(ctx_mw, kind, n):
+    """Lazy, strict and lookahead rules through the per-member walk: members
+    ending inside a block, on a block or group boundary, one past it."""
+    data = np.zeros(n, np.uint8) if kind == "zeros" else gpzip_amd.synth(kind, n, 11 + n)
+    t = O.match_find(data)
+    say("member walk", kind, n)
+    assert_tokens_equal(ctx_mw.walk(t, data), O.greedy_walk(t, data), f"lazy {kind} {n}")
+    assert_tokens_equal(ctx_mw.walk(t, strict=True), O.strict_walk(t), f"strict {kind} {n}")
+    assert_tokens_equal(ctx_mw.walk(t, data, rule=2), O.lookahead_walk(t, data), f"lookahead {kind} {n}")
+
+
+def test_member_walk_demotions_at_block_edges():
+    """Lazy demotions on the last lanes of a block and of a group (the rule
+    reads the next block's first tokens), decided by hand: position 64 k - 1
+    holds a match of 3 and position 64 k one of 5, so the first is demoted
+    to its literal byte."""
+    c = gpzip_amd.Context()
+    t = gpzip_amd.Tune()
+    t.walk_variant = 2
+    c.set_tune(t)
+    try:
+        n = 2048
+        data = gpzip_amd.synth("binmix", n, 5)
+        toks = data.astype(np.uint32)
+        for k in (1, 7, 8, 9, 16, 31):
+            p = 64 * k - 1
+            toks[p] = (3 << 16) | 1
+            toks[p + 1] = (5 << 16) | 2
+        say("member walk demotions")
+        assert_tokens_equal(c.walk(toks, data), O.greedy_walk(toks, data), "demotions")
+        assert_tokens_equal(c.walk(toks, data, rule=2), O.lookahead_walk(toks, data), "lookahead demotions")
+    finally:
+        c.close()
+
+
+def test_member_walk_forms_equal_oracle(ctx_mw):
+    """Plain, batched and edge-size chunks with the per-member walk equal the
+    oracle; a batch forced onto the per-segment walk gives the same bytes."""
+    kinds = ["text", "binmix", "log"]
+    sizes = [0, 1, 2, 3, 64, 4096, 131072, 131073, 700001, 2 << 20]
+    chunks = [gpzip_amd.synth(kinds[i % 3], max(s, 1), 300 + i)[:s] for i, s in enumerate(sizes)]
+    ref = [O.gzip_chunk(c) for c in chunks]
+    say("member walk plain")
+    for c, r in zip(chunks, ref):
+        assert ctx_mw.compress_chunk(c) == r, c.size
+    say("member walk batch")
+    assert ctx_mw.batch(chunks) == ref
+    seg = gpzip_amd.Context()
+    t = gpzip_amd.Tune()
+    t.walk_variant = 1
+    seg.set_tune(t)
+    try:
+        say("segment walk batch")
+        assert seg.batch(chunks) == ref
+    finally:
+        seg.close()
+    bad = gpzip_amd.Tune()
+    bad.walk_variant = 3
+    with pytest.raises(gpzip_amd.GpzError):
+        ctx_mw.set_tune(bad)
+
+
 @pytest.mark.parametrize("kind", ["text", "binmix", "log", "random"])
 def test_encode_block_equals_oracle(ctx, kind):
     n = 131072
@@ -233,12 +326,14 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     assert tune.k1_variant in (1, 2) and tune.emit_variant in (1, 2) and tune.k1_grid >= 1 and tune.k0_grid >= 1
     assert tune.k0_variant in (1, 2, 3) and tune.k1_seg_per_wg >= 1 and tune.k3_seg_per_wg >= 1
     assert tune.k3_grid >= 1 and tune.emit_grid >= 1 and t_ms > 0 and t_def > 0
+    assert tune.walk_variant in (1, 2)
     assert run(tune) == ref
     for v in (1, 2):
         for g in (1, 7, 256, 100000):
             t = gpzip_amd.Tune()
             t.k0_grid, t.k1_grid, t.k3_grid, t.emit_grid, t.k1_variant = g, g, g, max(1, g // 3), v
             t.emit_variant = 3 - v
+            t.walk_variant = 1 + (g & 1)
             say("shape", v, g)
             assert run(t) == ref
     # segment-table workgroup sizes and segments per workgroup
@@ -259,7 +354,7 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     finally:
         ctx.set_tune(None)
     assert ctx.get_tune().k1_grid == 0
-    for field, v in (("k1_variant", 3), ("emit_variant", 3), ("k0_variant", 4)):
+    for field, v in (("k1_variant", 3), ("emit_variant", 3), ("k0_variant", 4), ("walk_variant", 3)):
         bad = gpzip_amd.Tune()
         setattr(bad, field, v)
         with pytest.raises(gpzip_amd.GpzError):
