@@ -452,7 +452,10 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
     out->k0_variant = best.k0v == 0 ? 2 : (best.k0v == 1 ? 1 : 3);
     out->k1_seg_per_wg = best_k1_spw;
     out->k3_seg_per_wg = best_k3_spw;
-    out->walk_variant = best.walkv + 1;
+    // the walk is recorded only where the built-in rule (by member count)
+    // would not pick it: the same record then keeps the per-segment walk on
+    // the small steps of a server and the per-member walk on large batches
+    out->walk_variant = best.walkv == base.walkv ? 0 : best.walkv + 1;
     if (ms) {
         ms[0] = t1;
         ms[1] = t0;

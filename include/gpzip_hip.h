@@ -204,7 +204,8 @@ typedef struct gpz_tune {
     uint32_t walk_variant;  /* walk: 1 = per segment (walk summaries in the match-find kernel,
                                a chaining pass, one workgroup per segment), 2 = per member
                                (one wave walks each member in order); 0 = built-in (2 for
-                               batches of 512 members or more, else 1) */
+                               batches of 512 members or more, else 1; the autotuner records 0
+                               when the built-in choice is the faster one) */
     uint32_t reserved[2];
 } gpz_tune;
 
