@@ -1263,8 +1263,8 @@ __device__ __forceinline__ void walk_member(const Work &w, uint32_t m, uint32_t 
 #pragma unroll
         for (uint32_t k = 0; k < KM_G; k++) {
             const uint32_t bs = (gi * KM_G + k) << 6;
-            if (bs >= n) break;
-            if (e >= bs + 64u) continue;  // (uniform) a match jumps over the block
+            // (uniform) past the member end, or a match jumps over the block
+            if (e >= bs + 64u || bs >= n) continue;
             const bool in = bs + lane < n;
             const uint64_t valid = __ballot(in);
             const uint32_t t = in ? A.t[k] : 0u, len = t >> 16;
@@ -1281,23 +1281,24 @@ __device__ __forceinline__ void walk_member(const Work &w, uint32_t m, uint32_t 
                 const uint32_t t2 = lane_next(t1, bs + 65u < n ? rn1 : 0u);
                 dem = len != 0 && ((t1 >> 16) > len || (t2 >> 16) > len + 1);
             }
-            const uint32_t s = (len == 0 || dem) ? 1u : len;  // successor distance
+            const bool lit = len == 0 || dem;
+            const uint32_t jt = lane + (lit ? 1u : len);  // successor, block-relative
             // where a run of next-position successors stops: a match taken, or
             // lane 63 (its successor leaves the block either way)
-            const uint64_t stop = ~__ballot(s == 1) | (1ull << 63);
+            const uint64_t stop = ~__ballot(lit) | (1ull << 63);
             const uint32_t c0 = e - bs;
             uint32_t cur = c0;
             uint64_t jm = 0;  // positions where the walk takes a match
             do {
                 const uint32_t mm = cur + (uint32_t)__builtin_ctzll(stop >> cur);
-                jm |= 1ull << mm;
-                cur = mm + (uint32_t)__builtin_amdgcn_readlane((int)s, (int)mm);
+                asm("s_bitset1_b64 %0, %1" : "+s"(jm) : "s"(mm));
+                cur = (uint32_t)__builtin_amdgcn_readlane((int)jt, (int)mm);
             } while (cur < 64u);
             e = bs + cur;
             uint64_t V = valid & (~0ull << c0);
             if (jm & (~0ull >> 1)) {
-                // past a match taken at m the walk resumes at m + s(m)
-                const uint32_t r = wave_max_excl(((jm >> lane) & 1ull) ? lane + s : 0u);
+                // past a match taken at m the walk resumes at its successor
+                const uint32_t r = wave_max_excl(((jm >> lane) & 1ull) ? jt : 0u);
                 V &= __ballot(lane >= r);
             }
             uint32_t to = t;
