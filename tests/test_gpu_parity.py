@@ -564,6 +564,27 @@ def test_compress_chunk_with_1mib_members():
         c.close()
 
 
+@pytest.mark.parametrize("sub", [1 << 20, 4 << 20, 100003])
+def test_member_walk_large_and_odd_members(sub):
+    """The per-member walk on members of 1 MiB and 4 MiB (2,048 and 8,192 groups
+    of eight blocks per wave) and of 100,003 bytes (off the block grid): plain
+    and batched forms equal the oracle's split."""
+    c = gpzip_amd.Context(sub_chunk=sub)
+    t = gpzip_amd.Tune()
+    t.walk_variant = 2
+    c.set_tune(t)
+    try:
+        chunks = [gpzip_amd.synth(k, s, 80 + i) for i, (k, s) in
+                  enumerate([("text", (5 << 20) + 333), ("log", 4 << 20), ("binmix", (1 << 20) + 64)])]
+        say("member walk, members of", sub)
+        ref = [O.gzip_chunk(x, sub) for x in chunks]
+        for x, r in zip(chunks, ref):
+            assert c.compress_chunk(x) == r
+        assert c.batch(chunks) == ref
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("sub", [100003, 4000, 64])
 def test_member_sizes_off_the_segment_grid(sub):
     """Members of sizes that are not multiples of the 4 KiB segment (a
