@@ -137,6 +137,27 @@ def test_member_walk_equals_oracle(ctx_mw, kind, n):
     assert_tokens_equal(ctx_mw.walk(t, data, rule=2), O.lookahead_walk(t, data), f"lookahead {kind} {n}")
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_walks_on_arbitrary_token_streams(ctx, ctx_mw, seed):
+    """gpz_walk takes caller tokens: random streams of literals (some above
+    255), matches of length 1-258 (some running past the end) and distances
+    0-65535, in runs that make demotions common.  Both walks, all three rules,
+    equal the oracle's walks."""
+    rng = np.random.default_rng(seed)
+    for n in (1, 2, 63, 64, 65, 700, 4097, 20000, 70001):
+        kind = rng.integers(0, 4, n)
+        lens = np.where(kind == 0, 0, np.where(kind == 1, rng.integers(1, 3, n), rng.integers(3, 259, n)))
+        lens = np.where((kind == 3) & (rng.random(n) < 0.5), rng.integers(3, 9, n), lens)
+        vals = np.where(lens == 0, rng.integers(0, 300, n), rng.integers(0, 65536, n))
+        toks = ((lens.astype(np.uint32) << 16) | vals.astype(np.uint32)).astype(np.uint32)
+        data = rng.integers(0, 256, n, dtype=np.uint8)
+        say("arbitrary tokens", seed, n)
+        for c, name in ((ctx, "segment"), (ctx_mw, "member")):
+            assert_tokens_equal(c.walk(toks, data), O.greedy_walk(toks, data), f"{name} lazy {n}")
+            assert_tokens_equal(c.walk(toks, strict=True), O.strict_walk(toks), f"{name} strict {n}")
+            assert_tokens_equal(c.walk(toks, data, rule=2), O.lookahead_walk(toks, data), f"{name} lookahead {n}")
+
+
 def test_member_walk_demotions_at_block_edges():
     """Lazy demotions on the last lanes of a block and of a group (the rule
     reads the next block's first tokens), decided by hand: position 64 k - 1
