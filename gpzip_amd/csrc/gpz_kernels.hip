@@ -1286,13 +1286,16 @@ __device__ __forceinline__ void walk_member(const Work &w, uint32_t m, uint32_t 
             // where a run of next-position successors stops: a match taken, or
             // lane 63 (its successor leaves the block either way)
             const uint64_t stop = ~__ballot(lit) | (1ull << 63);
+            // the walk's next stop after a match taken here: the first stop at
+            // or past the successor, or the successor itself once it leaves
+            // the block (every stop mask has bit 63, so one exists)
+            const uint32_t kt = jt >= 64u ? jt : jt + (uint32_t)__builtin_ctzll(stop >> jt);
             const uint32_t c0 = e - bs;
-            uint32_t cur = c0;
+            uint32_t cur = c0 + (uint32_t)__builtin_ctzll(stop >> c0);
             uint64_t jm = 0;  // positions where the walk takes a match
             do {
-                const uint32_t mm = cur + (uint32_t)__builtin_ctzll(stop >> cur);
-                asm("s_bitset1_b64 %0, %1" : "+s"(jm) : "s"(mm));
-                cur = (uint32_t)__builtin_amdgcn_readlane((int)jt, (int)mm);
+                asm("s_bitset1_b64 %0, %1" : "+s"(jm) : "s"(cur));
+                cur = (uint32_t)__builtin_amdgcn_readlane((int)kt, (int)cur);
             } while (cur < 64u);
             e = bs + cur;
             uint64_t V = valid & (~0ull << c0);
