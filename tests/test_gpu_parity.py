@@ -851,3 +851,24 @@ def test_ratio_mode_forms_equal_oracle_and_are_smaller():
     finally:
         c.close()
         base.close()
+
+
+def test_gpzip_gpu_device_selection(monkeypatch):
+    """GPZIP_GPU as in the reference (codec-gpu/src/gpu/context.rs:119-155): an
+    index past the device count is GPZ_E_NODEV; "high", "low" and "0" select
+    the device, and an explicit gpz_config::device takes precedence."""
+    n = gpzip_amd.device_count()
+    monkeypatch.setenv("GPZIP_GPU", str(n + 3))
+    with pytest.raises(gpzip_amd.GpzError) as e:
+        gpzip_amd.Context()
+    assert e.value.status == gpzip_amd.GPZ_E_NODEV
+    c = gpzip_amd.Context(device=0)  # the config's device wins over the variable
+    c.close()
+    for v in ("high", "low", "0"):
+        monkeypatch.setenv("GPZIP_GPU", v)
+        c = gpzip_amd.Context()
+        try:
+            data = gpzip_amd.synth("text", 5000, 9)
+            assert c.compress_chunk(data) == O.gzip_chunk(data)
+        finally:
+            c.close()
