@@ -54,7 +54,7 @@ class Tune(C.Structure):
     _fields_ = [("k0_grid", C.c_uint32), ("k1_grid", C.c_uint32), ("k3_grid", C.c_uint32), ("emit_grid", C.c_uint32),
                 ("k1_variant", C.c_uint32), ("emit_variant", C.c_uint32), ("k0_variant", C.c_uint32),
                 ("k1_seg_per_wg", C.c_uint32), ("k3_seg_per_wg", C.c_uint32), ("walk_variant", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("walk_parts", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

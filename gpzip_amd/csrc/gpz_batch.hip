@@ -295,9 +295,11 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
         p->grids.emit = p->grids.huff = p->grids.chain = std::max<uint32_t>(1, p->pl.M);
     }
     if (tune) {
-        if (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3 || tune->walk_variant > 2) {
+        if (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3 || tune->walk_variant > 2 ||
+            tune->walk_parts == 3 || tune->walk_parts > 4) {
             delete p;
-            return fail(GPZ_E_INVAL, "k1_variant, emit_variant and walk_variant must be 0-2, k0_variant 0-3");
+            return fail(GPZ_E_INVAL,
+                        "k1_variant, emit_variant and walk_variant must be 0-2, k0_variant 0-3, walk_parts 0, 1, 2 or 4");
         }
         apply_tune(p->grids, *tune, p->pl.G, p->pl.M);
     } else if (flags & GPZ_PLAN_PERSISTENT) {
@@ -353,17 +355,22 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
     // segments per workgroup -> grid on this batch
     auto spw = [&](uint32_t n) { return (G + n - 1) / n; };
     uint32_t best_k1_spw = 0, best_k3_spw = 0;
-    {  // walk: per segment (summaries in match-find, chain, k_walk_emit) or per member;
-       // the interval holds match-find, the walk, Huffman and scan
+    {  // walk: per segment (summaries in match-find, chain, k_walk_emit) or per member
+       // on 1, 2 or 4 waves; the interval holds match-find, the walk, Huffman and scan
+        const uint32_t cs[][2] = {{0, 1}, {1, 1}, {1, 2}, {1, 4}};
         float bt = 1e30f;
-        for (uint32_t v = 0; v < 2 && !rc; v++) {
+        for (const auto &c : cs) {
             Grids g = best;
-            g.walkv = v;
+            g.walkv = c[0];
+            g.kmp = c[1];
+            g.km = (uint32_t)std::max<uint64_t>(1, ((uint64_t)M * c[1] + 3) / 4);
             float t;
             if ((rc = timed(g, 5, 3, &t))) break;
             if (t < bt) {
                 bt = t;
-                best.walkv = v;
+                best.walkv = g.walkv;
+                best.kmp = g.kmp;
+                best.km = g.km;
             }
         }
     }
@@ -456,6 +463,7 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
     // would not pick it: the same record then keeps the per-segment walk on
     // the small steps of a server and the per-member walk on large batches
     out->walk_variant = best.walkv == base.walkv ? 0 : best.walkv + 1;
+    out->walk_parts = best.walkv && best.kmp != base.kmp ? best.kmp : 0;
     if (ms) {
         ms[0] = t1;
         ms[1] = t0;

@@ -1210,32 +1210,33 @@ __device__ __forceinline__ uint32_t tok_bin2(uint32_t to) {
     return DSYM0 + (d <= 4 ? d - 1 : 2 * k + ((x >> (k - 1)) & 1u));
 }
 
-template <int RULE>
-__device__ __forceinline__ void walk_member(const Work &w, uint32_t m, uint32_t *hist, uint32_t *stg) {
+// The walk over blocks [b0, b1) of member md from entry e0 (member-relative, at
+// or past block b0's start).  Returns the first walked position at or past
+// block b1's start.  EMIT: the walked tokens are stored at out (in order; their
+// count in *cnt) and counted into hist; without it the walk only follows the
+// orbit.
+template <int RULE, bool EMIT>
+__device__ __forceinline__ uint32_t walk_span(const Work &w, const MemberDesc &md, uint32_t e0, uint32_t b0,
+                                              uint32_t b1, uint32_t *out, uint32_t *cnt, uint32_t *hist,
+                                              uint32_t *stg) {
     const uint32_t lane = threadIdx.x & 63u;
-    const MemberDesc md = load_member(w.mem, m);
     const uint32_t n = md.n;
-    if (n == 0) {  // an empty chunk's member: no tokens
-        for (uint32_t i = lane; i < NSYM; i += 64) w.hist[(uint64_t)m * NSYM + i] = 0;
-        if (lane == 0) w.ntok[m] = 0;
-        return;
+    if (b1 <= b0 || e0 >= (b1 << 6)) {  // (uniform) nothing of the span is walked
+        if (EMIT) *cnt = 0;
+        return e0;
     }
     const uint32_t *tk = w.tok + md.pos_base;
     const uint8_t *ib = w.in + md.in_off;
-    uint32_t *outp = w.walked + md.pos_base;
-    for (uint32_t i = lane; i < NSYM; i += 64) hist[i] = 0;
-    wave_lds_sync();
-    const uint32_t nb = (n + 63u) >> 6, ng = (nb + KM_G - 1) / KM_G;
+    const uint32_t ng = (b1 - b0 + KM_G - 1) / KM_G;
     // Loads are unconditional, their addresses clamped into the member (token
     // at the last position, input bytes at the last 8-byte boundary); what
     // lies past the member end is zeroed or ignored where it is used.
     const uint32_t xl = n - 1, rl = (n - 1) & ~7u;
     auto load = [&](WalkGroup &G, uint32_t gi) {
-        const uint32_t x0 = gi * (64u * KM_G) + lane;
+        const uint32_t p0 = (b0 + gi * KM_G) << 6;
 #pragma unroll
-        for (uint32_t k = 0; k <= KM_G; k++) G.t[k] = tk[min(x0 + 64u * k, xl)];
-        if (RULE != 1)
-            G.r = *reinterpret_cast<const uint2 *>(ib + min(gi * (64u * KM_G) + 8u * lane, rl));
+        for (uint32_t k = 0; k <= KM_G; k++) G.t[k] = tk[min(p0 + 64u * k + lane, xl)];
+        if (EMIT && RULE != 1) G.r = *reinterpret_cast<const uint2 *>(ib + min(p0 + 8u * lane, rl));
     };
     // Loads and stores share one counter on this target, so the compiler's
     // waits before the use of a loaded register drain it; there is one
@@ -1243,16 +1244,19 @@ __device__ __forceinline__ void walk_member(const Work &w, uint32_t m, uint32_t 
     // group earlier and the previous group's stores issued just as early:
     // walked tokens are staged in LDS (stg, 64 * KM_G words per wave) and
     // stored at the start of the next group.
-    uint32_t e = 0;      // next walked position (member-relative)
+    // next walked position (member-relative); uniform, kept in a scalar register
+    uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)e0);
     uint32_t outc = 0;   // tokens stored so far
     uint32_t scnt = 0;   // tokens staged
     auto flush = [&]() {
+        if (!EMIT) return;
         wave_lds_sync();
-        for (uint32_t i = lane; i < scnt; i += 64) outp[outc + i] = stg[i];
+        for (uint32_t i = lane; i < scnt; i += 64) out[outc + i] = stg[i];
         outc += scnt;
         scnt = 0;
         wave_lds_sync();  // the reads are done before the next group's writes
     };
+    const uint32_t pend = min(b1 << 6, n);  // the span's end (positions)
     WalkGroup A, B;  // A: the group being walked (loaded), B: the next (in flight)
     load(B, 0);
     for (uint32_t gi = 0; gi < ng; gi++) {
@@ -1262,11 +1266,10 @@ __device__ __forceinline__ void walk_member(const Work &w, uint32_t m, uint32_t 
         flush();
 #pragma unroll
         for (uint32_t k = 0; k < KM_G; k++) {
-            const uint32_t bs = (gi * KM_G + k) << 6;
-            // (uniform) past the member end, or a match jumps over the block
-            if (e >= bs + 64u || bs >= n) continue;
+            const uint32_t bs = (b0 + gi * KM_G + k) << 6;
+            // (uniform) past the span end, or a match jumps over the block
+            if (e >= bs + 64u || bs >= pend) continue;
             const bool in = bs + lane < n;
-            const uint64_t valid = __ballot(in);
             const uint32_t t = in ? A.t[k] : 0u, len = t >> 16;
             // tokens at x + 1 (and x + 2): the next lane, past lane 63 the next
             // block's first lanes; 0 past the member end
@@ -1298,7 +1301,8 @@ __device__ __forceinline__ void walk_member(const Work &w, uint32_t m, uint32_t 
                 cur = (uint32_t)__builtin_amdgcn_readlane((int)kt, (int)cur);
             } while (cur < 64u);
             e = bs + cur;
-            uint64_t V = valid & (~0ull << c0);
+            if (!EMIT) continue;
+            uint64_t V = __ballot(in) & (~0ull << c0);
             if (jm & (~0ull >> 1)) {
                 // past a match taken at m the walk resumes at its successor
                 const uint32_t r = wave_max_excl(((jm >> lane) & 1ull) ? jt : 0u);
@@ -1315,34 +1319,124 @@ __device__ __forceinline__ void walk_member(const Work &w, uint32_t m, uint32_t 
                 if (dem) to = rb;
             }
             bool match;
-            const uint32_t b1 = tok_bin1(to, match);
-            const uint32_t b2 = tok_bin2(to);
+            const uint32_t b1s = tok_bin1(to, match);
+            const uint32_t b2s = tok_bin2(to);
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(V >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)V, 0u));
             if ((V >> lane) & 1ull) {
                 stg[scnt + rank] = to;
-                atomicAdd(&hist[b1], 1u);
-                if (match) atomicAdd(&hist[b2], 1u);
+                atomicAdd(&hist[b1s], 1u);
+                if (match) atomicAdd(&hist[b2s], 1u);
             }
             scnt += (uint32_t)__popcll(V);
         }
     }
     flush();
-    for (uint32_t i = lane; i < NSYM; i += 64) w.hist[(uint64_t)m * NSYM + i] = i < HIST_USED ? hist[i] : 0u;
-    if (lane == 0) w.ntok[m] = outc;
-    wave_lds_sync();  // the histogram is cleared again for the next member
+    if (EMIT) *cnt = outc;
+    return e;
 }
 
+// Walk state of one part of a member, shared by the waves of a workgroup.
+struct PartState {
+    uint32_t guess;  // the part's entry as found by its warm-up walk
+    uint32_t exit;   // first walked position past the part
+    uint32_t count;  // tokens the part emits
+};
+
+// One member on P waves (P = 1, 2 or 4; 4 / P members per 256-thread
+// workgroup).  With P = 1 the member is walked in order by one wave.  With P >
+// 1 its blocks are cut into P parts, walked at once: wave q first walks the
+// KM_G blocks before its part without emitting, from that stretch's first
+// position, which gives a guess of the part's entry (walks from different
+// entries merge within a few positions on every corpus measured), then walks
+// and emits the part from the guess into a staging area (the member's stretch
+// of the segment tables, which the match finder no longer needs).  After a
+// barrier the guesses are checked in part order against the true entries --
+// part q's true entry is part q-1's exit -- and a part whose guess differs is
+// walked again from the true entry, so the result is exact.  The staged tokens
+// are then moved behind the previous parts' tokens, and the parts' histograms
+// summed.  Every wave runs the same barriers, whether or not its member slot
+// holds a member.
+template <int RULE, int P>
+__device__ __forceinline__ void walk_member(const Work &w, uint32_t M, uint32_t (*hs)[NSYM], uint32_t (*stg)[64 * KM_G],
+                                            PartState (*ps)[4]) {
+    constexpr uint32_t MPW = 4 / P;  // members per workgroup
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = uni(tid >> 6);
+    const uint32_t slot = wv / P, q = wv % P;
+    uint32_t *hist = hs[wv];
+    for (uint32_t g0 = blockIdx.x * MPW; g0 < M; g0 += gridDim.x * MPW) {
+        const uint32_t m = uni(g0 + slot);
+        const bool has = m < M;
+        MemberDesc md{};
+        if (has) md = load_member(w.mem, m);
+        const uint32_t n = has ? md.n : 0u;
+        const uint32_t nb = (n + 63u) >> 6;
+        const uint32_t pb = nb * q / P, pe = nb * (q + 1) / P;  // part q: blocks [pb, pe)
+        for (uint32_t i = lane; i < NSYM; i += 64) hist[i] = 0;
+        wave_lds_sync();
+        uint32_t *dst = has ? w.walked + md.pos_base : nullptr;
+        uint32_t *stage = (P > 1 && q > 0 && has) ? w.segtab + (uint64_t)md.seg_base * TAB_WORDS + (pb << 6) : dst;
+        uint32_t cnt = 0, ex = 0, guess = 0;
+        if (n) {
+            if (q > 0) {
+                const uint32_t wb = pb - min(KM_G, pb - nb * (q - 1) / P);
+                guess = walk_span<RULE, false>(w, md, wb << 6, wb, pb, nullptr, nullptr, hist, stg[wv]);
+            }
+            ex = walk_span<RULE, true>(w, md, max(guess, pb << 6), pb, pe, stage, &cnt, hist, stg[wv]);
+        }
+        if (P > 1) {
+            if (lane == 0) ps[slot][q] = PartState{max(guess, pb << 6), ex, cnt};
+            __syncthreads();
+            // check the guesses in part order; a wrong one is walked again
+            for (uint32_t r = 1; r < (uint32_t)P; r++) {
+                const uint32_t truth = uni(ps[slot][r - 1].exit), g = uni(ps[slot][r].guess);
+                if (n && q == r && truth != g) {
+                    for (uint32_t i = lane; i < NSYM; i += 64) hist[i] = 0;
+                    wave_lds_sync();
+                    ex = walk_span<RULE, true>(w, md, truth, pb, pe, stage, &cnt, hist, stg[wv]);
+                    if (lane == 0) ps[slot][q] = PartState{truth, ex, cnt};
+                }
+                __syncthreads();
+            }
+            // move the staged tokens behind the previous parts'
+            if (n && q > 0) {
+                uint32_t off = 0;
+                for (uint32_t r = 0; r < q; r++) off += uni(ps[slot][r].count);
+                for (uint32_t i = lane; i < cnt; i += 64) dst[off + i] = stage[i];
+            }
+            __syncthreads();  // every part's histogram is complete
+            if (has) {
+                for (uint32_t i = q * 64 + lane; i < NSYM; i += 64 * P) {
+                    uint32_t v = 0;
+#pragma unroll
+                    for (uint32_t r = 0; r < (uint32_t)P; r++) v += hs[slot * P + r][i];
+                    w.hist[(uint64_t)m * NSYM + i] = i < HIST_USED ? v : 0u;
+                }
+                if (q == 0 && lane == 0) {
+                    uint32_t tot = 0;
+                    for (uint32_t r = 0; r < (uint32_t)P; r++) tot += ps[slot][r].count;
+                    w.ntok[m] = tot;
+                }
+            }
+            __syncthreads();  // the histograms and part states are reused
+        } else if (has) {
+            wave_lds_sync();
+            for (uint32_t i = lane; i < NSYM; i += 64) w.hist[(uint64_t)m * NSYM + i] = i < HIST_USED ? hist[i] : 0u;
+            if (lane == 0) w.ntok[m] = cnt;
+            wave_lds_sync();  // the histogram is cleared again for the next member
+        }
+    }
+}
+
+template <int P>
 __global__ __launch_bounds__(KM_T) void k_walk_member(Work w) {
     __shared__ uint32_t hs[KM_T / 64][NSYM];
     __shared__ uint32_t stg[KM_T / 64][64 * KM_G];
-    const uint32_t wv = threadIdx.x >> 6;
+    __shared__ PartState ps[4][4];
     const uint32_t M = uni(w.counts->nmem);
     const uint32_t rule = uni(w.strict);
-    for (uint32_t m = uni(blockIdx.x * (KM_T / 64) + wv); m < M; m += gridDim.x * (KM_T / 64)) {
-        if (rule == 0) walk_member<0>(w, m, hs[wv], stg[wv]);
-        else if (rule == 1) walk_member<1>(w, m, hs[wv], stg[wv]);
-        else walk_member<2>(w, m, hs[wv], stg[wv]);
-    }
+    if (rule == 0) walk_member<0, P>(w, M, hs, stg, ps);
+    else if (rule == 1) walk_member<1, P>(w, M, hs, stg, ps);
+    else walk_member<2, P>(w, M, hs, stg, ps);
 }
 
 // Histogram of caller-supplied walked tokens (stage-level gpz_encode_block).
@@ -2182,8 +2276,13 @@ void launch_chain(const Work &w, uint32_t grid, hipStream_t st) {
 void launch_walk_emit(const Work &w, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_walk_emit, dim3(grid), dim3(K3_T), 0, st, w);
 }
-void launch_walk_member(const Work &w, uint32_t grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_walk_member, dim3(grid), dim3(KM_T), 0, st, w);
+void launch_walk_member(const Work &w, uint32_t grid, int parts, hipStream_t st) {
+    if (parts == 4)
+        hipLaunchKernelGGL(k_walk_member<4>, dim3(grid), dim3(KM_T), 0, st, w);
+    else if (parts == 2)
+        hipLaunchKernelGGL(k_walk_member<2>, dim3(grid), dim3(KM_T), 0, st, w);
+    else
+        hipLaunchKernelGGL(k_walk_member<1>, dim3(grid), dim3(KM_T), 0, st, w);
 }
 void launch_hist_tokens(const Work &w, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_hist_tokens, dim3(grid), dim3(1024), 0, st, w);

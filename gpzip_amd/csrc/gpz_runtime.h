@@ -56,11 +56,12 @@ struct Grids {
     uint32_t k1v = 0;    // match-find variant: 0 two workgroups per CU, 1 one
     uint32_t emitv = 0;  // bit-emission variant: 0 1024 threads per member, 1 256
     uint32_t k0v = 0;    // segment-table workgroup: 0 512 threads, 1 256, 2 1024
-    uint32_t km = 1;     // grid of the per-member walk (four members per workgroup)
+    uint32_t km = 1;     // grid of the per-member walk (four waves per workgroup)
     uint32_t walkv = 0;  // walk: 0 per segment (summaries, chain, k_walk_emit), 1 per member
+    uint32_t kmp = 1;    // per-member walk: waves (parts) per member, 1, 2 or 4
     bool operator==(const Grids &o) const {
         return k0 == o.k0 && k1 == o.k1 && k3 == o.k3 && chain == o.chain && huff == o.huff && emit == o.emit &&
-               k1v == o.k1v && emitv == o.emitv && k0v == o.k0v && km == o.km && walkv == o.walkv;
+               k1v == o.k1v && emitv == o.emitv && k0v == o.k0v && km == o.km && walkv == o.walkv && kmp == o.kmp;
     }
 };
 

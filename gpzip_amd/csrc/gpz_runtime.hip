@@ -33,7 +33,7 @@ void launch_match_find(const Work &w, uint32_t grid, int summary, int variant, h
 void launch_walk_summary(const Work &w, uint32_t grid, hipStream_t st);
 void launch_chain(const Work &w, uint32_t grid, hipStream_t st);
 void launch_walk_emit(const Work &w, uint32_t grid, hipStream_t st);
-void launch_walk_member(const Work &w, uint32_t grid, hipStream_t st);
+void launch_walk_member(const Work &w, uint32_t grid, int parts, hipStream_t st);
 void launch_hist_tokens(const Work &w, uint32_t grid, hipStream_t st);
 void launch_huffman(const Work &w, uint32_t grid, hipStream_t st);
 void launch_scan(const Work &w, hipStream_t st);
@@ -335,9 +335,12 @@ Grids grids_for(const Caps &c, int n_cu) {
     g.huff = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 16));
     g.emit = (uint32_t)std::max<uint64_t>(1, c.M);
     // the per-member walk runs a member's blocks in sequence: it pays once a
-    // batch has enough members to fill the chip, the per-segment walk below that
-    g.km = (uint32_t)std::max<uint64_t>(1, (c.M + 3) / 4);
+    // batch has enough members to fill the chip, the per-segment walk below
+    // that; with fewer than 8,192 members each member is cut into 2 or 4
+    // parts walked by as many waves
     g.walkv = c.M >= 512 ? 1 : 0;
+    g.kmp = c.M >= 8192 ? 1 : (c.M >= 4096 ? 2 : 4);
+    g.km = (uint32_t)std::max<uint64_t>(1, (c.M * g.kmp + 3) / 4);
     return g;
 }
 
@@ -353,6 +356,10 @@ void apply_tune(Grids &g, const gpz_tune &t, uint64_t G, uint64_t M) {
     if (t.emit_variant == 1 || t.emit_variant == 2) g.emitv = t.emit_variant - 1;
     if (t.k0_variant >= 1 && t.k0_variant <= 3) g.k0v = t.k0_variant == 2 ? 0 : (t.k0_variant == 1 ? 1 : 2);
     if (t.walk_variant == 1 || t.walk_variant == 2) g.walkv = t.walk_variant - 1;
+    if (t.walk_parts == 1 || t.walk_parts == 2 || t.walk_parts == 4) {
+        g.kmp = t.walk_parts;
+        g.km = (uint32_t)std::max<uint64_t>(1, (M * g.kmp + 3) / 4);
+    }
     // a record tuned on a larger batch must not launch workgroups with no
     // segment or member to take
     auto cap = [](uint32_t v, uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(v, n)); };
@@ -373,7 +380,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
             break;
         case MODE_WALK:
             if (g.walkv) {
-                launch_walk_member(w, g.km, st);
+                launch_walk_member(w, g.km, (int)g.kmp, st);
             } else {
                 launch_walk_summary(w, g.k1, st);
                 launch_chain(w, g.chain, st);
@@ -396,7 +403,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
                 // per-member walk: no summaries, no chaining pass
                 launch_match_find(w, g.k1, 0, g.k1v, st);
                 if (ev) hipEventRecord(ev[1], st);
-                launch_walk_member(w, g.km, st);
+                launch_walk_member(w, g.km, (int)g.kmp, st);
             } else {
                 if (w.strict == 2) {
                     // ratio mode: the walk rule reads two tokens ahead, so the
@@ -519,8 +526,12 @@ const gpz_tune &ctx_tune(gpz_ctx *c) { return c->tune; }
 // one.
 static Grids lane_grids(gpz_ctx *ctx, const Caps &c) {
     Grids g = grids_for(c, ctx->n_cu);
-    const uint32_t wv = ctx->tune.walk_variant;
+    const uint32_t wv = ctx->tune.walk_variant, wp = ctx->tune.walk_parts;
     if (wv == 1 || wv == 2) g.walkv = wv - 1;
+    if (wp == 1 || wp == 2 || wp == 4) {
+        g.kmp = wp;
+        g.km = (uint32_t)std::max<uint64_t>(1, (c.M * wp + 3) / 4);
+    }
     return g;
 }
 Lane *ctx_acquire(gpz_ctx *c) { return c->acquire(); }
@@ -591,8 +602,10 @@ extern "C" {
 int gpz_set_tune(gpz_ctx *ctx, const gpz_tune *tune) {
     clear_error();
     if (!ctx) return fail(GPZ_E_INVAL, "null context");
-    if (tune && (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3 || tune->walk_variant > 2))
-        return fail(GPZ_E_INVAL, "k1_variant, emit_variant and walk_variant must be 0-2, k0_variant 0-3");
+    if (tune && (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3 || tune->walk_variant > 2 ||
+                 tune->walk_parts == 3 || tune->walk_parts > 4))
+        return fail(GPZ_E_INVAL,
+                    "k1_variant, emit_variant and walk_variant must be 0-2, k0_variant 0-3, walk_parts 0, 1, 2 or 4");
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (tune) ctx->tune = *tune;
     else ctx->tune = gpz_tune{};

@@ -206,7 +206,10 @@ typedef struct gpz_tune {
                                (one wave walks each member in order); 0 = built-in (2 for
                                batches of 512 members or more, else 1; the autotuner records 0
                                when the built-in choice is the faster one) */
-    uint32_t reserved[2];
+    uint32_t walk_parts;    /* per-member walk: waves per member, 1, 2 or 4 (a member's blocks
+                               cut into that many parts walked at once); 0 = built-in (4 below
+                               4,096 members, 2 below 8,192, else 1) */
+    uint32_t reserved[1];
 } gpz_tune;
 
 /* Tuning record used by every batched launch of this context (gpz_batch_*,

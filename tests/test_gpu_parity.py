@@ -209,6 +209,70 @@ def test_member_walk_forms_equal_oracle(ctx_mw):
         ctx_mw.set_tune(bad)
 
 
+@pytest.mark.parametrize("parts", [1, 2, 4])
+def test_member_walk_parts_equal_oracle(parts):
+    """The per-member walk with each member cut into 1, 2 or 4 parts walked at
+    once.  A stream of a literal followed by matches of length 2 never lets a
+    walk from a block start (even) meet the true walk (odd), so every part's
+    warm-up guess is wrong and every part is walked again from the true
+    entry; the stream with the literal removed has right guesses.  Reference
+    cases, random token streams, matched corpora and whole chunks (plain and
+    batched) equal the oracle."""
+    c = gpzip_amd.Context()
+    t = gpzip_amd.Tune()
+    t.walk_variant, t.walk_parts = 2, parts
+    c.set_tune(t)
+    try:
+        for n in (1, 64, 65, 129, 700, 4097, 70001):
+            for lead in (True, False):
+                toks = np.full(n, (2 << 16) | 1, np.uint32)
+                if lead:
+                    toks[0] = 65
+                data = np.full(n, 65, np.uint8)
+                say("parts", parts, "len-2 stream", n, lead)
+                exp = O.strict_walk(toks)
+                assert_tokens_equal(c.walk(toks, strict=True), exp, f"strict {n}")
+                assert_tokens_equal(c.walk(toks, data), O.greedy_walk(toks, data), f"lazy {n}")
+                assert_tokens_equal(c.walk(toks, data, rule=2), O.lookahead_walk(toks, data), f"lookahead {n}")
+        for case in GOLD["walk_strict"]:
+            if "all_literals" not in case:
+                assert_tokens_equal(c.walk(pack(case["tokens"]), strict=True), pack(case["expect"]), case["name"])
+        rng = np.random.default_rng(parts)
+        for n in (3, 200, 5000, 70001):
+            lens = np.where(rng.random(n) < 0.4, 0, rng.integers(1, 259, n))
+            vals = np.where(lens == 0, rng.integers(0, 256, n), rng.integers(1, 32769, n))
+            toks = ((lens.astype(np.uint32) << 16) | vals.astype(np.uint32)).astype(np.uint32)
+            data = rng.integers(0, 256, n, dtype=np.uint8)
+            say("parts", parts, "random stream", n)
+            assert_tokens_equal(c.walk(toks, data), O.greedy_walk(toks, data), f"lazy random {n}")
+            assert_tokens_equal(c.walk(toks, strict=True), O.strict_walk(toks), f"strict random {n}")
+            assert_tokens_equal(c.walk(toks, data, rule=2), O.lookahead_walk(toks, data), f"lookahead random {n}")
+        for kind in ("text", "binmix", "log"):
+            data = gpzip_amd.synth(kind, 131072, 17)
+            tk = O.match_find(data)
+            say("parts", parts, "corpus", kind)
+            assert_tokens_equal(c.walk(tk, data), O.greedy_walk(tk, data), f"lazy {kind}")
+            assert_tokens_equal(c.walk(tk, data, rule=2), O.lookahead_walk(tk, data), f"lookahead {kind}")
+        sizes = [0, 1, 64, 4096, 131072, 131073, 700001]
+        chunks = [gpzip_amd.synth(("text", "binmix", "log")[i % 3], max(s, 1), 500 + i)[:s]
+                  for i, s in enumerate(sizes)]
+        ref = [O.gzip_chunk(x) for x in chunks]
+        say("parts", parts, "chunks")
+        for x, r in zip(chunks, ref):
+            assert c.compress_chunk(x) == r, x.size
+        assert c.batch(chunks) == ref
+    finally:
+        c.close()
+    bad = gpzip_amd.Tune()
+    bad.walk_parts = 3
+    with pytest.raises(gpzip_amd.GpzError):
+        ctx_bad = gpzip_amd.Context()
+        try:
+            ctx_bad.set_tune(bad)
+        finally:
+            ctx_bad.close()
+
+
 @pytest.mark.parametrize("kind", ["text", "binmix", "log", "random"])
 def test_encode_block_equals_oracle(ctx, kind):
     n = 131072
@@ -347,7 +411,7 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     assert tune.k1_variant in (1, 2) and tune.emit_variant in (1, 2) and tune.k1_grid >= 1 and tune.k0_grid >= 1
     assert tune.k0_variant in (1, 2, 3) and tune.k1_seg_per_wg >= 1 and tune.k3_seg_per_wg >= 1
     assert tune.k3_grid >= 1 and tune.emit_grid >= 1 and t_ms > 0 and t_def > 0
-    assert tune.walk_variant in (0, 1, 2)
+    assert tune.walk_variant in (0, 1, 2) and tune.walk_parts in (0, 1, 2, 4)
     assert run(tune) == ref
     for v in (1, 2):
         for g in (1, 7, 256, 100000):
@@ -355,6 +419,7 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
             t.k0_grid, t.k1_grid, t.k3_grid, t.emit_grid, t.k1_variant = g, g, g, max(1, g // 3), v
             t.emit_variant = 3 - v
             t.walk_variant = 1 + (g & 1)
+            t.walk_parts = (1, 2, 4)[g % 3]
             say("shape", v, g)
             assert run(t) == ref
     # segment-table workgroup sizes and segments per workgroup
@@ -375,7 +440,8 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     finally:
         ctx.set_tune(None)
     assert ctx.get_tune().k1_grid == 0
-    for field, v in (("k1_variant", 3), ("emit_variant", 3), ("k0_variant", 4), ("walk_variant", 3)):
+    for field, v in (("k1_variant", 3), ("emit_variant", 3), ("k0_variant", 4), ("walk_variant", 3),
+                     ("walk_parts", 3), ("walk_parts", 8)):
         bad = gpzip_amd.Tune()
         setattr(bad, field, v)
         with pytest.raises(gpzip_amd.GpzError):
