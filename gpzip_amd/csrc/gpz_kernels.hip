@@ -1225,17 +1225,24 @@ __device__ __forceinline__ uint32_t walk_span(const Work &w, const MemberDesc &m
         if (EMIT) *cnt = 0;
         return e0;
     }
-    const uint32_t *tk = w.tok + md.pos_base;
+    // the span's addresses are computed here, not hoisted over an earlier
+    // walk (where they would hold registers through its whole loop)
+    b0 = uni(b0);
+    asm volatile("" : "+s"(b0));
+    const uint8_t *tk = reinterpret_cast<const uint8_t *>(w.tok + md.pos_base);
     const uint8_t *ib = w.in + md.in_off;
     const uint32_t ng = (b1 - b0 + KM_G - 1) / KM_G;
     // Loads are unconditional, their addresses clamped into the member (token
     // at the last position, input bytes at the last 8-byte boundary); what
-    // lies past the member end is zeroed or ignored where it is used.
+    // lies past the member end is zeroed or ignored where it is used.  Token
+    // offsets are 32-bit byte offsets (a member holds at most 2^24 tokens), so
+    // each load takes a scalar base and one offset register.
     const uint32_t xl = n - 1, rl = (n - 1) & ~7u;
     auto load = [&](WalkGroup &G, uint32_t gi) {
         const uint32_t p0 = (b0 + gi * KM_G) << 6;
 #pragma unroll
-        for (uint32_t k = 0; k <= KM_G; k++) G.t[k] = tk[min(p0 + 64u * k + lane, xl)];
+        for (uint32_t k = 0; k <= KM_G; k++)
+            G.t[k] = *reinterpret_cast<const uint32_t *>(tk + (min(p0 + 64u * k + lane, xl) << 2));
         if (EMIT && RULE != 1) G.r = *reinterpret_cast<const uint2 *>(ib + min(p0 + 8u * lane, rl));
     };
     // Loads and stores share one counter on this target, so the compiler's
@@ -1375,28 +1382,33 @@ __device__ __forceinline__ void walk_member(const Work &w, uint32_t M, uint32_t 
         wave_lds_sync();
         uint32_t *dst = has ? w.walked + md.pos_base : nullptr;
         uint32_t *stage = (P > 1 && q > 0 && has) ? w.segtab + (uint64_t)md.seg_base * TAB_WORDS + (pb << 6) : dst;
-        uint32_t cnt = 0, ex = 0, guess = 0;
-        if (n) {
-            if (q > 0) {
-                const uint32_t wb = pb - min(KM_G, pb - nb * (q - 1) / P);
-                guess = walk_span<RULE, false>(w, md, wb << 6, wb, pb, nullptr, nullptr, hist, stg[wv]);
-            }
-            ex = walk_span<RULE, true>(w, md, max(guess, pb << 6), pb, pe, stage, &cnt, hist, stg[wv]);
+        uint32_t cnt = 0, entry = pb << 6;
+        if (P > 1 && n && q > 0) {
+            const uint32_t wb = pb - min(KM_G, pb - nb * (q - 1) / P);
+            entry = max(entry, walk_span<RULE, false>(w, md, wb << 6, wb, pb, nullptr, nullptr, hist, stg[wv]));
         }
-        if (P > 1) {
-            if (lane == 0) ps[slot][q] = PartState{max(guess, pb << 6), ex, cnt};
-            __syncthreads();
-            // check the guesses in part order; a wrong one is walked again
-            for (uint32_t r = 1; r < (uint32_t)P; r++) {
-                const uint32_t truth = uni(ps[slot][r - 1].exit), g = uni(ps[slot][r].guess);
-                if (n && q == r && truth != g) {
+        // round 0: every part walks from its guess; round r > 0: part r walks
+        // again from its true entry (part r-1's exit) if its guess was wrong
+        for (uint32_t r = 0; r < (uint32_t)P; r++) {
+            bool go = r == 0;
+            if (P > 1 && r > 0) {
+                const uint32_t truth = uni(ps[slot][r - 1].exit);
+                go = q == r && truth != entry;
+                if (go) {
+                    entry = truth;
                     for (uint32_t i = lane; i < NSYM; i += 64) hist[i] = 0;
                     wave_lds_sync();
-                    ex = walk_span<RULE, true>(w, md, truth, pb, pe, stage, &cnt, hist, stg[wv]);
-                    if (lane == 0) ps[slot][q] = PartState{truth, ex, cnt};
                 }
-                __syncthreads();
             }
+            if (go && n) {
+                const uint32_t ex = walk_span<RULE, true>(w, md, entry, pb, pe, stage, &cnt, hist, stg[wv]);
+                if (P > 1 && lane == 0) ps[slot][q] = PartState{entry, ex, cnt};
+            } else if (P > 1 && go && lane == 0) {
+                ps[slot][q] = PartState{entry, entry, 0};
+            }
+            if (P > 1) __syncthreads();
+        }
+        if (P > 1) {
             // move the staged tokens behind the previous parts'
             if (n && q > 0) {
                 uint32_t off = 0;
@@ -1427,8 +1439,11 @@ __device__ __forceinline__ void walk_member(const Work &w, uint32_t M, uint32_t 
     }
 }
 
+// Eight waves per SIMD (at most 64 registers of each kind; a few scalar values
+// move to vector-register lanes): the walk is latency-bound, and 2,048 members
+// on four waves each then run in one round on the 1,024 SIMDs.
 template <int P>
-__global__ __launch_bounds__(KM_T) void k_walk_member(Work w) {
+__global__ __launch_bounds__(KM_T) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_walk_member(Work w) {
     __shared__ uint32_t hs[KM_T / 64][NSYM];
     __shared__ uint32_t stg[KM_T / 64][64 * KM_G];
     __shared__ PartState ps[4][4];

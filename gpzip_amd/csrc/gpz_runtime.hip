@@ -336,10 +336,10 @@ Grids grids_for(const Caps &c, int n_cu) {
     g.emit = (uint32_t)std::max<uint64_t>(1, c.M);
     // the per-member walk runs a member's blocks in sequence: it pays once a
     // batch has enough members to fill the chip, the per-segment walk below
-    // that; with fewer than 8,192 members each member is cut into 2 or 4
-    // parts walked by as many waves
+    // that; each member is cut into four parts walked by four waves (faster
+    // than one or two parts on 2,048 and on 21,444 members)
     g.walkv = c.M >= 512 ? 1 : 0;
-    g.kmp = c.M >= 8192 ? 1 : (c.M >= 4096 ? 2 : 4);
+    g.kmp = 4;
     g.km = (uint32_t)std::max<uint64_t>(1, (c.M * g.kmp + 3) / 4);
     return g;
 }

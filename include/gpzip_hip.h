@@ -203,12 +203,12 @@ typedef struct gpz_tune {
     uint32_t k3_seg_per_wg; /* walk/emit-tokens segments per workgroup; 0 = use k3_grid */
     uint32_t walk_variant;  /* walk: 1 = per segment (walk summaries in the match-find kernel,
                                a chaining pass, one workgroup per segment), 2 = per member
-                               (one wave walks each member in order); 0 = built-in (2 for
+                               (each member walked in order by one to four waves); 0 = built-in (2 for
                                batches of 512 members or more, else 1; the autotuner records 0
                                when the built-in choice is the faster one) */
     uint32_t walk_parts;    /* per-member walk: waves per member, 1, 2 or 4 (a member's blocks
-                               cut into that many parts walked at once); 0 = built-in (4 below
-                               4,096 members, 2 below 8,192, else 1) */
+                               cut into that many parts walked at once); 0 = built-in (4; the
+                               autotuner records 0 when 4 is the fastest) */
     uint32_t reserved[1];
 } gpz_tune;
 
