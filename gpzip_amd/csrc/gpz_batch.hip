@@ -295,11 +295,12 @@ int gpz_plan_create(gpz_ctx *ctx, const gpz_desc *descs, size_t n, uint32_t flag
         p->grids.emit = p->grids.huff = p->grids.chain = std::max<uint32_t>(1, p->pl.M);
     }
     if (tune) {
-        if (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3 || tune->walk_variant > 2 ||
+        if (tune->k1_variant > 2 || tune->emit_variant > 4 || tune->k0_variant > 3 || tune->walk_variant > 2 ||
             tune->walk_parts == 3 || tune->walk_parts > 4) {
             delete p;
             return fail(GPZ_E_INVAL,
-                        "k1_variant, emit_variant and walk_variant must be 0-2, k0_variant 0-3, walk_parts 0, 1, 2 or 4");
+                        "k1_variant and walk_variant must be 0-2, k0_variant 0-3, emit_variant 0-4, "
+                    "walk_parts 0, 1, 2 or 4");
         }
         apply_tune(p->grids, *tune, p->pl.G, p->pl.M);
     } else if (flags & GPZ_PLAN_PERSISTENT) {
@@ -429,8 +430,15 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
         }
     }
     if (!rc) {  // bit emission: workgroup size x persistent grid
-        const Cand cs[] = {{cap(cu, M), 0}, {cap((uint64_t)cu * 2, M), 0}, {cap((uint64_t)cu * 4, M), 0}, {M, 0},
-                           {cap((uint64_t)cu * 8, M), 1}, {cap((uint64_t)cu * 16, M), 1}, {M, 1}};
+        const Cand cs[] = {{cap(cu, M), 0},
+                           {cap((uint64_t)cu * 2, M), 0},
+                           {cap((uint64_t)cu * 4, M), 0},
+                           {M, 0},
+                           {cap((uint64_t)cu * 8, M), 1},
+                           {cap((uint64_t)cu * 16, M), 1},
+                           {M, 1},
+                           {M, 2},  // one workgroup per slot (grid: segments + members)
+                           {M, 3}};
         float bt = 1e30f;
         for (const Cand &c : cs) {
             Grids g = best;

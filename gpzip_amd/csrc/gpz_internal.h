@@ -68,6 +68,9 @@ struct Work {
     uint64_t *chunk_off;        // [C]
     uint64_t *chunk_len;        // [C]
     uint64_t *total;            // [1]
+    uint64_t *em_state;         // [G] k_emit_tiled look-back: flag << 62 | bits
+    uint64_t *em_join;          // [G] k_emit_tiled: parts added << 32 | bits of a word two slots share
+    uint32_t *em_tick;          // [M] k_emit_tiled: token slots taken of each member
     uint8_t *out;               // output bytes (4-byte aligned)
     uint32_t strict;            // walk rule: 0 lazy (production), 1 strict, 2 lazy with a two-token
                                 // lookahead (the opt-in ratio mode)

@@ -1744,6 +1744,11 @@ __global__ __launch_bounds__(64) void k_huffman(Work w) {
     for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
         const MemberDesc md = load_member(w.mem, m);
         for (uint32_t i = lane; i < NSYM; i += 64) freq[i] = w.hist[(uint64_t)m * NSYM + i];
+        for (uint32_t s = lane; s < md.nseg; s += 64) {  // k_emit_tiled's slot state
+            w.em_state[md.seg_base + s] = 0;
+            w.em_join[md.seg_base + s] = 0;
+        }
+        if (lane == 0) w.em_tick[m] = 0;
         for (uint32_t i = lane; i < 20; i += 64) mfreq[i] = 0;
         for (uint32_t i = lane; i < PREFIX_WORDS + 2; i += 64) pbuf[i] = 0;
         __syncthreads();
@@ -2118,6 +2123,241 @@ __global__ __launch_bounds__(BT) void k_emit(Work w) {
 }
 
 // ---------------------------------------------------------------------------
+// k_emit_tiled: bit emission with one workgroup per slot of a member, every
+// slot at once.  Member m has a header slot (the gzip header and dynamic
+// block header bits that k_huffman assembled) and one token slot per segment.
+// Token slot t = 1 .. nseg holds tokens [ET (t-1), ET t) while that range is
+// full (t <= nfull = min(ntok / ET, nseg - 1)); the last slot (t = nseg) holds
+// the tokens from ET nfull to ntok, the end-of-block code, the pad to a byte
+// and the gzip trailer; the slots between are empty.  A token slot's bit
+// offset is the header's bits plus those of the member's full slots before
+// it, each of which publishes its count as soon as it has it.  Every
+// nonempty slot spans at least 32 bits (the header
+// at least 80, a full slot at least ET, the last at least 64), so a 32-bit
+// output word is shared by at most two slots, neighbours among the nonempty
+// ones: both add their parts into one atomic word, and the second to arrive
+// writes the output word.  Words at the member's two ends are written byte by
+// byte within the member.  A member's token-slot workgroups take their slots
+// from a per-member ticket in the order they start, so a slot only ever waits
+// on slots whose workgroups run.  k_huffman clears the tickets and the slot
+// state.
+constexpr uint32_t ET = 4096;                   // tokens per token slot
+constexpr uint32_t ET_BUF = ET * 48 / 32 + 16;  // slot bits (<= 48 per token) + offset + trailer
+constexpr uint64_t ES_AGG = 1ull << 62, ES_VAL = (1ull << 62) - 1;
+
+// Word wi of the output, only the bytes inside [A, B).
+__device__ __forceinline__ void store_word_in(uint8_t *out, uint64_t wi, uint32_t v, uint64_t A, uint64_t B) {
+    const uint64_t b0 = wi * 4;
+    if (b0 >= A && b0 + 4 <= B) {
+        reinterpret_cast<uint32_t *>(out)[wi] = v;
+    } else {
+        for (int k = 0; k < 4; k++) {
+            const uint64_t a = b0 + k;
+            if (a >= A && a < B) out[a] = (uint8_t)(v >> (8 * k));
+        }
+    }
+}
+
+// The word shared with a neighbouring slot.  The two slots' parts occupy
+// disjoint bits, so adding a part is or-ing it: each slot adds 1 << 32 plus
+// its part to join[key] in one atomic, and the one that finds the other's
+// part there (count 1) writes the word.  The data travels in the atomic, so
+// no fence is needed.
+__device__ __forceinline__ void combine_word(const Work &w, uint32_t key, uint32_t part, uint64_t wi, uint64_t A,
+                                             uint64_t B) {
+    const uint64_t old = atomicAdd(reinterpret_cast<unsigned long long *>(&w.em_join[key]),
+                                   (unsigned long long)((1ull << 32) | part));
+    if ((old >> 32) == 1) store_word_in(w.out, wi, (uint32_t)old | part, A, B);
+}
+
+// Code bits of one token: up to four fields (code, extra, distance code, extra).
+__device__ __forceinline__ uint32_t token_fields(const uint32_t *ctab, uint32_t t, uint32_t v[4], uint32_t nb[4]) {
+    const uint32_t len = t >> 16, x = t & 0xFFFFu;
+    v[1] = v[2] = v[3] = 0;
+    nb[1] = nb[2] = nb[3] = 0;
+    if (len == 0) {
+        const uint32_t c = ctab[x & 0xFF];
+        v[0] = c & 0xFFFF;
+        nb[0] = c >> 16;
+    } else {
+        const uint32_t li = len_idx(len);
+        const uint32_t c = ctab[257 + li];
+        v[0] = c & 0xFFFF;
+        nb[0] = c >> 16;
+        v[1] = len - c_len_base[li];
+        nb[1] = len_extra(li);
+        const uint32_t ds = dist_sym(x);
+        const uint32_t dc = ctab[DSYM0 + ds];
+        v[2] = dc & 0xFFFF;
+        nb[2] = dc >> 16;
+        v[3] = x - c_dist_base[ds];
+        nb[3] = dist_extra(ds);
+    }
+    return nb[0] + nb[1] + nb[2] + nb[3];
+}
+
+__device__ __forceinline__ void or_bits_at(uint32_t *buf, uint32_t pos, uint32_t v, uint32_t nb) {
+    if (!nb) return;
+    const uint32_t wi = pos >> 5, sh = pos & 31;
+    atomicOr(&buf[wi], v << sh);
+    if (sh + nb > 32) atomicOr(&buf[wi + 1], v >> (32 - sh));
+}
+
+template <int BT>
+__global__ __launch_bounds__(BT) void k_emit_tiled(Work w) {
+    constexpr uint32_t TPT = ET / BT;  // tokens per thread
+    __shared__ uint32_t ctab[NSYM];
+    __shared__ uint32_t buf[ET_BUF];
+    __shared__ uint32_t tmp[32];
+    __shared__ uint32_t s_slot;
+    __shared__ uint64_t s_pre;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t G = uni(w.counts->nseg), M = uni(w.counts->nmem);
+    const uint32_t b = blockIdx.x;
+    if (b >= G + M) return;
+    // workgroups [0, G) take the token slots, one per segment, in the order
+    // they start (a ticket per member); workgroups [G, G + M) the headers
+    const bool hdr = b >= G;
+    const uint32_t m = hdr ? b - G : uni(w.seg_mem[b]);
+    const MemberDesc md = load_member(w.mem, m);
+    const uint32_t nseg = md.nseg, nt = uni(w.ntok[m]);
+    if (!hdr) {
+        if (tid == 0) s_slot = atomicAdd(&w.em_tick[m], 1u);
+        __syncthreads();
+    }
+    const uint32_t t = hdr ? 0u : uni(s_slot) + 1;
+    const uint32_t u = md.seg_base + t - 1;  // a token slot's segment (its state index)
+    const uint32_t nfull = nseg ? min(nt / ET, nseg - 1) : 0u;
+    const bool last = t == nseg;  // the member's final slot (the header of an empty member)
+    if (!hdr && !last && t > nfull) return;  // an empty slot
+    const uint64_t A = w.mem_off[m], B = A + w.mem_bytes[m];
+    for (uint32_t i = tid; i < NSYM; i += BT) ctab[i] = w.codes[(uint64_t)m * NSYM + i];
+    const uint32_t pbits = uni(w.prefix_bits[m]);
+
+    // this thread's tokens and their bit count
+    const uint32_t t0 = hdr ? 0u : (last ? nfull * ET : (t - 1) * ET);
+    const uint32_t t1 = hdr ? 0u : (last ? nt : t * ET);
+    const uint32_t *tk = w.walked + md.pos_base;
+    uint32_t tq[TPT];
+#pragma unroll
+    for (uint32_t k = 0; k < TPT; k += 4) {
+        const uint32_t i = t0 + tid * TPT + k;
+        uint4 q = make_uint4(0, 0, 0, 0);
+        if (i + 4 <= t1) {
+            q = *reinterpret_cast<const uint4 *>(tk + i);
+        } else {
+            if (i < t1) q.x = tk[i];
+            if (i + 1 < t1) q.y = tk[i + 1];
+            if (i + 2 < t1) q.z = tk[i + 2];
+        }
+        tq[k] = q.x;
+        tq[k + 1] = q.y;
+        tq[k + 2] = q.z;
+        tq[k + 3] = q.w;
+    }
+    __syncthreads();  // ctab
+    uint32_t mybits = 0;
+    {
+        uint32_t v[4], nb[4];
+#pragma unroll
+        for (uint32_t k = 0; k < TPT; k++)
+            if (t0 + tid * TPT + k < t1) mybits += token_fields(ctab, tq[k], v, nb);
+    }
+    uint32_t tile_bits;
+    const uint32_t rel = block_scan<BT>(mybits, tmp, &tile_bits);
+    if (hdr) tile_bits = pbits;
+
+    // bits of the member before this slot: the header's and those of the full
+    // slots before it, which each publish their count as soon as they have
+    // it; the first wave reads up to 64 of them at once
+    uint64_t pre = 0;
+    if (!hdr) {
+        if (tid < 64) {
+            uint64_t *st = w.em_state + md.seg_base;  // slot t at st[t - 1]
+            if (!last && tid == 0)
+                __hip_atomic_store(&st[t - 1], ES_AGG | tile_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t acc = 0;
+            for (uint32_t j0 = 1, top = last ? nfull : t - 1; j0 <= top; j0 += 64) {
+                const uint32_t j = j0 + tid;
+                uint64_t s = 0;
+                if (j <= top) {
+                    // slot j's workgroup has started (tickets), so it
+                    // publishes; the bound only keeps a broken state from
+                    // hanging the queue
+                    for (uint32_t spin = 0; spin < (1u << 22); spin++) {
+                        s = __hip_atomic_load(&st[j - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (s >> 62) break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                uint64_t v = s & ES_VAL;
+                for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+                acc += v;
+            }
+            if (tid == 0) s_pre = acc + pbits;
+        }
+        __syncthreads();
+        pre = s_pre;
+    }
+
+    const uint64_t S = A * 8 + pre;  // absolute bit position of the slot's first bit
+    const uint32_t off0 = (uint32_t)(S & 31);
+    uint32_t all_bits = tile_bits, padded = 0;
+    const uint32_t eob = ctab[256], eb = eob >> 16;
+    if (last) {
+        const uint64_t e = S + tile_bits;
+        padded = (uint32_t)(((e + eb + 7) & ~7ull) - e);
+        all_bits += padded + (w.gzip ? 64 : 0);
+    }
+    const uint32_t nw = (off0 + all_bits + 31) >> 5;
+    for (uint32_t i = tid; i <= nw; i += BT) buf[i] = 0;
+    __syncthreads();
+    if (hdr) {
+        const uint32_t *pw = w.prefix + (uint64_t)m * PREFIX_WORDS;
+        for (uint32_t i = tid; i * 32 < pbits; i += BT) or_bits_at(buf, off0 + i * 32, pw[i], min(32u, pbits - i * 32));
+    } else {
+        uint32_t r = off0 + rel;
+#pragma unroll
+        for (uint32_t k = 0; k < TPT; k++) {
+            if (t0 + tid * TPT + k >= t1) break;
+            uint32_t v[4], nb[4];
+            token_fields(ctab, tq[k], v, nb);
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                or_bits_at(buf, r, v[f], nb[f]);
+                r += nb[f];
+            }
+        }
+    }
+    if (last && tid == 0) {
+        const uint32_t e = off0 + tile_bits;
+        or_bits_at(buf, e, eob & 0xFFFF, eb);
+        if (w.gzip) {
+            or_bits_at(buf, e + padded, w.mem_crc[m], 32);
+            or_bits_at(buf, e + padded + 32, md.n, 32);
+        }
+    }
+    __syncthreads();
+    // words: the member's first and last words byte by byte, a word shared
+    // with a neighbouring slot through combine_word, the rest whole
+    const uint64_t w0 = S >> 5;
+    const bool head_shared = !hdr && off0 != 0;
+    const bool tail_shared = !last && ((off0 + all_bits) & 31) != 0;
+    for (uint32_t i = tid; i < nw; i += BT) {
+        if ((i == 0 && head_shared) || (i + 1 == nw && tail_shared)) continue;
+        store_word_in(w.out, w0 + i, buf[i], A, B);
+    }
+    if (tid == 0) {
+        if (head_shared) combine_word(w, u, buf[0], w0, A, B);  // (never the header)
+        if (tail_shared) {
+            // the next nonempty slot of the member
+            const uint32_t tn = (t < nfull) ? t + 1 : nseg;
+            combine_word(w, md.seg_base + tn - 1, buf[nw - 1], w0 + nw - 1, A, B);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_plan_one: the member and segment tables of one chunk of len bytes cut into
 // members of `sub` bytes, written on the device (the host's build_plan, in
 // closed form: every member but the last is full).  For callers whose
@@ -2312,8 +2552,12 @@ void launch_plan_one(const Work &w, uint32_t len, uint32_t sub, hipStream_t st) 
 void launch_plan_batch(const Work &w, hipStream_t st) {
     hipLaunchKernelGGL(k_plan_batch, dim3(1), dim3(1024), 0, st, w);
 }
-void launch_emit(const Work &w, uint32_t grid, int variant, hipStream_t st) {
-    if (variant == 1)
+void launch_emit(const Work &w, uint32_t grid, uint32_t slots, int variant, hipStream_t st) {
+    if (variant == 3)
+        hipLaunchKernelGGL(k_emit_tiled<256>, dim3(slots), dim3(256), 0, st, w);
+    else if (variant == 2)
+        hipLaunchKernelGGL(k_emit_tiled<1024>, dim3(slots), dim3(1024), 0, st, w);
+    else if (variant == 1)
         hipLaunchKernelGGL(k_emit<256>, dim3(grid), dim3(256), 0, st, w);
     else
         hipLaunchKernelGGL(k_emit<1024>, dim3(grid), dim3(1024), 0, st, w);

@@ -48,20 +48,22 @@ struct Caps {
 struct Layout {
     uint64_t mem, seg_mem, chunk_mem0, counts, cdesc, desc_end;
     uint64_t tok, segtab, seg_ntab, wsum, seg_crc, seg_entry, seg_tokbase, walked, hist, ntok, codes, prefix, prefix_bits;
-    uint64_t mem_bytes, mem_crc, mem_off, chunk_off, chunk_len, total, out, end;
+    uint64_t mem_bytes, mem_crc, mem_off, chunk_off, chunk_len, total, em_state, em_join, em_tick, out, end;
 };
 
 struct Grids {
     uint32_t k0 = 1, k1 = 1, k3 = 1, chain = 1, huff = 1, emit = 1;
     uint32_t k1v = 0;    // match-find variant: 0 two workgroups per CU, 1 one
-    uint32_t emitv = 0;  // bit-emission variant: 0 1024 threads per member, 1 256
+    uint32_t emitv = 0;  // bit-emission variant: 0 1024 threads per member, 1 256; 2 / 3 one
+                         // workgroup of 1024 / 256 threads per slot (k_emit_tiled)
+    uint32_t eslots = 1; // k_emit_tiled grid: segments + members
     uint32_t k0v = 0;    // segment-table workgroup: 0 512 threads, 1 256, 2 1024
     uint32_t km = 1;     // grid of the per-member walk (four waves per workgroup)
     uint32_t walkv = 0;  // walk: 0 per segment (summaries, chain, k_walk_emit), 1 per member
     uint32_t kmp = 1;    // per-member walk: waves (parts) per member, 1, 2 or 4
     bool operator==(const Grids &o) const {
         return k0 == o.k0 && k1 == o.k1 && k3 == o.k3 && chain == o.chain && huff == o.huff && emit == o.emit &&
-               k1v == o.k1v && emitv == o.emitv && k0v == o.k0v && km == o.km && walkv == o.walkv && kmp == o.kmp;
+               k1v == o.k1v && emitv == o.emitv && eslots == o.eslots && k0v == o.k0v && km == o.km && walkv == o.walkv && kmp == o.kmp;
     }
 };
 

@@ -37,7 +37,7 @@ void launch_walk_member(const Work &w, uint32_t grid, int parts, hipStream_t st)
 void launch_hist_tokens(const Work &w, uint32_t grid, hipStream_t st);
 void launch_huffman(const Work &w, uint32_t grid, hipStream_t st);
 void launch_scan(const Work &w, hipStream_t st);
-void launch_emit(const Work &w, uint32_t grid, int variant, hipStream_t st);
+void launch_emit(const Work &w, uint32_t grid, uint32_t slots, int variant, hipStream_t st);
 void launch_plan_one(const Work &w, uint32_t len, uint32_t sub, hipStream_t st);
 void launch_plan_batch(const Work &w, hipStream_t st);
 
@@ -220,6 +220,9 @@ Layout layout_for(const Caps &c) {
     L.chunk_off = carve(cur, c.C * 8);
     L.chunk_len = carve(cur, c.C * 8);
     L.total = carve(cur, 8);
+    L.em_state = carve(cur, c.G * 8);
+    L.em_join = carve(cur, c.G * 8);
+    L.em_tick = carve(cur, c.M * 4);
     L.out = carve(cur, c.out);
     L.end = cur;
     return L;
@@ -251,6 +254,9 @@ Work work_at(const Layout &L, uint8_t *b) {
     w.chunk_off = (uint64_t *)(b + L.chunk_off);
     w.chunk_len = (uint64_t *)(b + L.chunk_len);
     w.total = (uint64_t *)(b + L.total);
+    w.em_state = (uint64_t *)(b + L.em_state);
+    w.em_join = (uint64_t *)(b + L.em_join);
+    w.em_tick = (uint32_t *)(b + L.em_tick);
     w.out = b + L.out;
     w.window = 32768;
     return w;
@@ -334,6 +340,11 @@ Grids grids_for(const Caps &c, int n_cu) {
     g.chain = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 16));
     g.huff = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.M, (uint64_t)n_cu * 16));
     g.emit = (uint32_t)std::max<uint64_t>(1, c.M);
+    g.eslots = (uint32_t)std::max<uint64_t>(1, c.G + c.M);
+    // bit emission: few members (a plain chunk's 16) leave one workgroup per
+    // member latency-bound, so every 4,096-token slot gets its own; large
+    // batches keep one workgroup per member
+    g.emitv = c.M < 512 ? 2 : 0;
     // the per-member walk runs a member's blocks in sequence: it pays once a
     // batch has enough members to fill the chip, the per-segment walk below
     // that; each member is cut into four parts walked by four waves (faster
@@ -353,7 +364,7 @@ void apply_tune(Grids &g, const gpz_tune &t, uint64_t G, uint64_t M) {
     if (t.k1_seg_per_wg) g.k1 = (uint32_t)((G + t.k1_seg_per_wg - 1) / t.k1_seg_per_wg);
     if (t.k3_seg_per_wg) g.k3 = (uint32_t)((G + t.k3_seg_per_wg - 1) / t.k3_seg_per_wg);
     if (t.k1_variant == 1 || t.k1_variant == 2) g.k1v = t.k1_variant - 1;
-    if (t.emit_variant == 1 || t.emit_variant == 2) g.emitv = t.emit_variant - 1;
+    if (t.emit_variant >= 1 && t.emit_variant <= 4) g.emitv = t.emit_variant - 1;
     if (t.k0_variant >= 1 && t.k0_variant <= 3) g.k0v = t.k0_variant == 2 ? 0 : (t.k0_variant == 1 ? 1 : 2);
     if (t.walk_variant == 1 || t.walk_variant == 2) g.walkv = t.walk_variant - 1;
     if (t.walk_parts == 1 || t.walk_parts == 2 || t.walk_parts == 4) {
@@ -392,7 +403,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
             launch_huffman(w, g.huff, st);
             launch_scan(w, st);
             if (ev) hipEventRecord(ev[3], st);
-            launch_emit(w, g.emit, g.emitv, st);
+            launch_emit(w, g.emit, g.eslots, g.emitv, st);
             if (ev) hipEventRecord(ev[4], st);
             break;
         default:  // MODE_FULL
@@ -421,7 +432,7 @@ void enqueue(const Work &w, const Grids &g, int mode, hipStream_t st, hipEvent_t
             launch_huffman(w, g.huff, st);
             launch_scan(w, st);
             if (ev) hipEventRecord(ev[3], st);
-            launch_emit(w, g.emit, g.emitv, st);
+            launch_emit(w, g.emit, g.eslots, g.emitv, st);
             if (ev) hipEventRecord(ev[4], st);
             break;
     }
@@ -602,10 +613,11 @@ extern "C" {
 int gpz_set_tune(gpz_ctx *ctx, const gpz_tune *tune) {
     clear_error();
     if (!ctx) return fail(GPZ_E_INVAL, "null context");
-    if (tune && (tune->k1_variant > 2 || tune->emit_variant > 2 || tune->k0_variant > 3 || tune->walk_variant > 2 ||
+    if (tune && (tune->k1_variant > 2 || tune->emit_variant > 4 || tune->k0_variant > 3 || tune->walk_variant > 2 ||
                  tune->walk_parts == 3 || tune->walk_parts > 4))
         return fail(GPZ_E_INVAL,
-                    "k1_variant, emit_variant and walk_variant must be 0-2, k0_variant 0-3, walk_parts 0, 1, 2 or 4");
+                    "k1_variant and walk_variant must be 0-2, k0_variant 0-3, emit_variant 0-4, "
+                    "walk_parts 0, 1, 2 or 4");
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (tune) ctx->tune = *tune;
     else ctx->tune = gpz_tune{};

@@ -195,8 +195,11 @@ typedef struct gpz_tune {
     uint32_t emit_grid;  /* persistent grid of the bit-emission kernel */
     uint32_t k1_variant; /* match-find occupancy: 1 = two workgroups per CU (64 VGPRs),
                             2 = one workgroup per CU (128 VGPRs); 0 = built-in */
-    uint32_t emit_variant; /* bit-emission workgroup: 1 = 1024 threads per member,
-                              2 = 256 threads (eight members per CU); 0 = built-in (1) */
+    uint32_t emit_variant; /* bit emission: 1 = one 1024-thread workgroup per member,
+                              2 = one 256-thread workgroup per member (eight members per CU),
+                              3 / 4 = one 1024 / 256-thread workgroup per 4,096-token slot of
+                              a member, every slot at once (emit_grid unused); 0 = built-in (3
+                              below 512 members, else 1) */
     uint32_t k0_variant;   /* segment-table workgroup: 1 = 256 threads (16 positions each),
                               2 = 512 (8), 3 = 1024 (4); 0 = built-in (2) */
     uint32_t k1_seg_per_wg; /* match-find segments per workgroup; 0 = use k1_grid */

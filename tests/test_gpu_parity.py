@@ -361,6 +361,75 @@ def test_device_plan_across_tiles_and_smaller_batches(ctx):
     assert ctx.batch(small) == [res[1023], res[5], res[1024]]
 
 
+def _plan_outputs(ctx, chunks, tune, flags):
+    """Outputs of one gpz_plan over `chunks` (resident on the device) with
+    the given tuning record and plan flags, run twice (the second run checks
+    that per-run state starts clean)."""
+    import torch
+
+    offs, o = [], 0
+    for c in chunks:
+        offs.append(o)
+        o += (c.size + 255) // 256 * 256
+    host = np.zeros(o + 256, np.uint8)
+    for off, c in zip(offs, chunks):
+        host[off:off + c.size] = c
+    d_in = torch.from_numpy(host).to("cuda:0")
+    arr = (gpzip_amd.Desc * len(chunks))()
+    for i, (a, c) in enumerate(zip(offs, chunks)):
+        arr[i].offset, arr[i].len = a, c.size
+    L = ctx.L
+    h = gpzip_amd.C.c_void_p()
+    t = gpzip_amd.C.byref(tune) if tune is not None else None
+    gpzip_amd._check(L.gpz_plan_create(ctx.h, arr, len(chunks), flags, t, gpzip_amd.C.byref(h)))
+    try:
+        cap = L.gpz_plan_out_capacity(h)
+        runs = []
+        for _ in range(2):
+            d_out = torch.zeros(cap + 256, dtype=torch.uint8, device="cuda:0")
+            gpzip_amd._check(L.gpz_plan_run(h, gpzip_amd.C.c_void_p(d_in.data_ptr()),
+                                            gpzip_amd.C.c_void_p(d_out.data_ptr())))
+            gpzip_amd._check(L.gpz_plan_sync(h))
+            off = np.zeros(len(chunks), np.uint64)
+            ln = np.zeros(len(chunks), np.uint64)
+            tot = gpzip_amd.C.c_uint64()
+            gpzip_amd._check(L.gpz_plan_result(h, off.ctypes.data_as(gpzip_amd.u64p),
+                                               ln.ctypes.data_as(gpzip_amd.u64p), gpzip_amd.C.byref(tot)))
+            out = d_out.cpu().numpy()
+            runs.append([out[int(a):int(a) + int(b)].tobytes() for a, b in zip(off, ln)])
+        assert runs[0] == runs[1]
+        return runs[0]
+    finally:
+        L.gpz_plan_destroy(h)
+
+
+@pytest.mark.parametrize("variant", [3, 4])
+def test_tiled_emit_equals_oracle(ctx, variant):
+    """Bit emission with one workgroup per 4,096-token slot (look-back
+    offsets, words shared by two slots written by the second to arrive):
+    empty and tiny members, members whose token count is an exact multiple
+    of 4,096 (random bytes: one literal per position, so the last slot is
+    full), members with a few tokens and many empty slots (zeros), and text /
+    binmix / log members, in the persistent, direct and graph plan forms,
+    equal the oracle."""
+    rng = np.random.default_rng(variant)
+    chunks = [np.zeros(0, np.uint8), np.array([7], np.uint8), rng.integers(0, 256, 100, dtype=np.uint8),
+              rng.integers(0, 256, 4096, dtype=np.uint8), rng.integers(0, 256, 4097, dtype=np.uint8),
+              rng.integers(0, 256, 8192, dtype=np.uint8), rng.integers(0, 256, 131072, dtype=np.uint8),
+              np.zeros(131072, np.uint8), np.zeros(300001, np.uint8)]
+    chunks += [gpzip_amd.synth(k, s, 40 + i) for i, (k, s) in
+               enumerate([("text", 131072), ("binmix", 131073), ("log", 700001), ("text", 3 << 20),
+                          ("binmix", 2 << 20)])]
+    ref = [O.gzip_chunk(c) for c in chunks]
+    t = gpzip_amd.Tune()
+    t.emit_variant = variant
+    for flags in (gpzip_amd.GPZ_PLAN_PERSISTENT, 0, gpzip_amd.GPZ_PLAN_GRAPH):
+        say("tiled emit", variant, "flags", flags)
+        got = _plan_outputs(ctx, chunks, t, flags)
+        for i, (g, r) in enumerate(zip(got, ref)):
+            assert g == r, f"chunk {i} ({chunks[i].size} bytes), plan flags {flags}"
+
+
 def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     """The autotuner returns a valid record; every launch shape it can pick
     (and both match-find occupancy variants) gives the same bytes."""
@@ -408,7 +477,7 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     say("autotune")
     probe = torch.zeros(4 * o + (1 << 20), dtype=torch.uint8, device="cuda:0")
     tune, t_ms, t_def = ctx.autotune(descs, d_in.data_ptr(), probe.data_ptr())
-    assert tune.k1_variant in (1, 2) and tune.emit_variant in (1, 2) and tune.k1_grid >= 1 and tune.k0_grid >= 1
+    assert tune.k1_variant in (1, 2) and tune.emit_variant in (1, 2, 3, 4) and tune.k1_grid >= 1 and tune.k0_grid >= 1
     assert tune.k0_variant in (1, 2, 3) and tune.k1_seg_per_wg >= 1 and tune.k3_seg_per_wg >= 1
     assert tune.k3_grid >= 1 and tune.emit_grid >= 1 and t_ms > 0 and t_def > 0
     assert tune.walk_variant in (0, 1, 2) and tune.walk_parts in (0, 1, 2, 4)
@@ -417,7 +486,7 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
         for g in (1, 7, 256, 100000):
             t = gpzip_amd.Tune()
             t.k0_grid, t.k1_grid, t.k3_grid, t.emit_grid, t.k1_variant = g, g, g, max(1, g // 3), v
-            t.emit_variant = 3 - v
+            t.emit_variant = (2, 1, 3, 4)[(g + v) % 4]
             t.walk_variant = 1 + (g & 1)
             t.walk_parts = (1, 2, 4)[g % 3]
             say("shape", v, g)
@@ -440,7 +509,7 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     finally:
         ctx.set_tune(None)
     assert ctx.get_tune().k1_grid == 0
-    for field, v in (("k1_variant", 3), ("emit_variant", 3), ("k0_variant", 4), ("walk_variant", 3),
+    for field, v in (("k1_variant", 3), ("emit_variant", 5), ("k0_variant", 4), ("walk_variant", 3),
                      ("walk_parts", 3), ("walk_parts", 8)):
         bad = gpzip_amd.Tune()
         setattr(bad, field, v)
