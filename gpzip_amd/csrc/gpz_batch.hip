@@ -463,7 +463,9 @@ int gpz_autotune(gpz_ctx *ctx, const gpz_desc *descs, size_t n, const void *d_in
     out->k3_grid = best.k3;
     out->emit_grid = best.emit;
     out->k1_variant = best.k1v + 1;
-    out->emit_variant = best.emitv + 1;
+    // recorded only where it differs from the built-in choice by member count,
+    // which keeps a server's small steps on the per-slot emission
+    out->emit_variant = best.emitv == base.emitv ? 0 : best.emitv + 1;
     out->k0_variant = best.k0v == 0 ? 2 : (best.k0v == 1 ? 1 : 3);
     out->k1_seg_per_wg = best_k1_spw;
     out->k3_seg_per_wg = best_k3_spw;

@@ -343,8 +343,8 @@ Grids grids_for(const Caps &c, int n_cu) {
     g.eslots = (uint32_t)std::max<uint64_t>(1, c.G + c.M);
     // bit emission: few members (a plain chunk's 16) leave one workgroup per
     // member latency-bound, so every 4,096-token slot gets its own; large
-    // batches keep one workgroup per member
-    g.emitv = c.M < 512 ? 2 : 0;
+    // batches keep one 256-thread workgroup per member
+    g.emitv = c.M < 512 ? 2 : 1;
     // the per-member walk runs a member's blocks in sequence: it pays once a
     // batch has enough members to fill the chip, the per-segment walk below
     // that; each member is cut into four parts walked by four waves (faster

@@ -199,7 +199,8 @@ typedef struct gpz_tune {
                               2 = one 256-thread workgroup per member (eight members per CU),
                               3 / 4 = one 1024 / 256-thread workgroup per 4,096-token slot of
                               a member, every slot at once (emit_grid unused); 0 = built-in (3
-                              below 512 members, else 1) */
+                              below 512 members, else 2; the autotuner records 0 when the
+                              built-in choice is the faster one) */
     uint32_t k0_variant;   /* segment-table workgroup: 1 = 256 threads (16 positions each),
                               2 = 512 (8), 3 = 1024 (4); 0 = built-in (2) */
     uint32_t k1_seg_per_wg; /* match-find segments per workgroup; 0 = use k1_grid */

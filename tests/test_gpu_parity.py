@@ -477,7 +477,7 @@ def test_autotune_and_tuned_shapes_equal_builtin(ctx):
     say("autotune")
     probe = torch.zeros(4 * o + (1 << 20), dtype=torch.uint8, device="cuda:0")
     tune, t_ms, t_def = ctx.autotune(descs, d_in.data_ptr(), probe.data_ptr())
-    assert tune.k1_variant in (1, 2) and tune.emit_variant in (1, 2, 3, 4) and tune.k1_grid >= 1 and tune.k0_grid >= 1
+    assert tune.k1_variant in (1, 2) and tune.emit_variant in (0, 1, 2, 3, 4) and tune.k1_grid >= 1 and tune.k0_grid >= 1
     assert tune.k0_variant in (1, 2, 3) and tune.k1_seg_per_wg >= 1 and tune.k3_seg_per_wg >= 1
     assert tune.k3_grid >= 1 and tune.emit_grid >= 1 and t_ms > 0 and t_def > 0
     assert tune.walk_variant in (0, 1, 2) and tune.walk_parts in (0, 1, 2, 4)
