@@ -2,22 +2,29 @@
 //
 // Pipeline per batch (one launch each, all gridded so one instance can be
 // replayed from a HIP graph for any batch that fits the workspace):
-//   k_match_find  one 1024-thread workgroup per 4 KiB segment: exact
-//                 segmented-hash candidates for the segment's positions from an
-//                 LDS copy of its 36 KiB window, candidate verification and
-//                 extension, per-position tokens, the segment's walk summary
-//                 (exit/count for every possible entry offset, by pointer
-//                 doubling) and its raw CRC-32.
-//   k_chain       one wave per member: chains the 32 segment summaries to get
-//                 each segment's walk entry and output token base.
-//   k_walk_emit   one workgroup per segment: marks the walked positions from
-//                 the known entry (pointer doubling), compacts them into the
-//                 token stream and accumulates the member histogram.
+//   k_plan_batch  (device-built tables) member and segment tables from the
+//                 chunk descriptors.
+//   k_seg_table   per 4 KiB segment: the hash set as a bitmap with ranks, the
+//                 first/last position of each hash, and the raw CRC-32.
+//   k_match_find  one 1024-thread workgroup per segment: exact segmented-hash
+//                 candidates for the segment's positions from an LDS copy of
+//                 its 36 KiB window and of each back segment's table,
+//                 candidate verification and extension, per-position tokens
+//                 and, for the per-segment walk, the segment's walk summary
+//                 (exit/count for every possible entry offset).
+//   walk          per segment below 512 members: k_chain (one wave per member
+//                 chains the summaries into each segment's entry and token
+//                 base) and k_walk_emit (one workgroup per segment marks the
+//                 walked positions, compacts them, counts the histogram); per
+//                 member from 512 members up: k_walk_member (each member cut
+//                 into parts walked at once by the waves of a workgroup).
 //   k_huffman     one wave per member: exact heap-order Huffman code lengths,
 //                 canonical codes, dynamic block header, member size and CRC.
 //   k_scan        member output offsets and per-chunk offsets/lengths.
-//   k_emit        one workgroup per member: per-token bit lengths, workgroup
-//                 scan, LDS word assembly, coalesced stores of the member.
+//   emission      k_emit (one workgroup per member: per-token bit lengths,
+//                 workgroup scan, LDS word assembly, coalesced stores) from 512
+//                 members up; k_emit_tiled (one workgroup per 4,096-token slot
+//                 of a member, all at once) below.
 //
 // Reference semantics restated here (paths relative to the reference repo,
 // crates/gpzip- prefix dropped):
@@ -2131,16 +2138,15 @@ __global__ __launch_bounds__(BT) void k_emit(Work w) {
 // the tokens from ET nfull to ntok, the end-of-block code, the pad to a byte
 // and the gzip trailer; the slots between are empty.  A token slot's bit
 // offset is the header's bits plus those of the member's full slots before
-// it, each of which publishes its count as soon as it has it.  Every
-// nonempty slot spans at least 32 bits (the header
-// at least 80, a full slot at least ET, the last at least 64), so a 32-bit
-// output word is shared by at most two slots, neighbours among the nonempty
-// ones: both add their parts into one atomic word, and the second to arrive
-// writes the output word.  Words at the member's two ends are written byte by
-// byte within the member.  A member's token-slot workgroups take their slots
-// from a per-member ticket in the order they start, so a slot only ever waits
-// on slots whose workgroups run.  k_huffman clears the tickets and the slot
-// state.
+// it, each of which publishes its count as soon as it has it.  Every nonempty
+// slot spans at least 32 bits (the header at least 80, a full slot at least
+// ET, the last at least 64), so a 32-bit output word is shared by at most two
+// slots, neighbours among the nonempty ones: both add their parts into one
+// atomic word, and the second to arrive writes the output word.  Words at the
+// member's two ends are written byte by byte within the member.  A member's
+// token-slot workgroups take their slots from a per-member ticket in the
+// order they start, so a slot only ever waits on slots whose workgroups run.
+// k_huffman clears the tickets and the slot state.
 constexpr uint32_t ET = 4096;                   // tokens per token slot
 constexpr uint32_t ET_BUF = ET * 48 / 32 + 16;  // slot bits (<= 48 per token) + offset + trailer
 constexpr uint64_t ES_AGG = 1ull << 62, ES_VAL = (1ull << 62) - 1;
