@@ -64,3 +64,32 @@ def test_synth_deterministic():
     assert lg.count(b"\n") > 300 and max(lg) < 128
     s = gpzip_amd.loguniform_sizes(3, 4096, 4096, 4 << 20)
     assert s.min() >= 4096 and s.max() <= 4 << 20
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of the header's structs have the C sizes and field
+    offsets (a small C program built with gcc against include/gpzip_hip.h
+    prints them)."""
+    import shutil
+    import subprocess
+
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    mirrors = {"gpz_config": gpzip_amd.Config, "gpz_desc": gpzip_amd.Desc, "gpz_tune": gpzip_amd.Tune,
+               "gpz_server_config": gpzip_amd.ServerConfig}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "gpzip_hip.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
