@@ -1155,7 +1155,7 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
 }
 
 // ---------------------------------------------------------------------------
-// k_walk_member: one wave walks a whole member in position order, 64
+// k_walk_member: a wave walks a member (or a part of one) in position order, 64
 // positions (one block) at a time, from position 0 -- the reference's walk
 // (lz77.rs:260-284; strict rule walk.rs:512-525; rule 2 the opt-in ratio
 // mode).  Per block the lanes compute their successors; the orbit through the
@@ -1169,10 +1169,10 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
 // whole at the end (no global atomics, no clearing pass).  Without
 // per-segment walk summaries there is no chaining pass either.  Tokens and
 // input bytes are loaded one group of KM_G blocks ahead.  This is the walk of
-// large batches: a member's blocks run in sequence, so a batch needs many
+// large batches: a part's blocks run in sequence, so a batch needs many
 // members to fill the chip.
 
-constexpr int KM_T = 256;     // four waves per workgroup, each its own member
+constexpr int KM_T = 256;     // four waves per workgroup (four members on one wave each, two on two, one on four)
 constexpr uint32_t KM_G = 8;  // blocks of 64 positions per group
 constexpr uint32_t HIST_USED = 316;  // litlen 0..285, distance 286..315; 319 takes invalid tokens
 
