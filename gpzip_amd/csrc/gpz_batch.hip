@@ -60,6 +60,10 @@ static void copy_pieces(uint8_t *dst, const uint64_t *dst_off, const uint8_t *co
 }
 
 // One large buffer copied by up to 8 threads.
+// Host copies and transfers of a batch go in pieces of this size, each
+// overlapping the next piece's.
+constexpr uint64_t PIPE_BYTES = 32ull << 20;
+
 static void copy_bytes(uint8_t *dst, const uint8_t *src, uint64_t n) {
     constexpr uint64_t PIECE = 4ull << 20;
     const uint64_t np = (n + PIECE - 1) / PIECE;
@@ -124,8 +128,18 @@ int lane_batch_launch(gpz_ctx *ctx, Lane *ln, const Plan &pl, const uint8_t *con
     if ((rc = ln->din.ensure(pl.in_bytes))) return rc;
     if ((rc = ln->hin.ensure(pl.in_bytes))) return rc;
     HIPCHK(hipSetDevice(ctx_device(ctx)));
-    copy_pieces((uint8_t *)ln->hin.p, pl.chunk_dev_off.data(), ptrs, lens, pl.C, copy_threads);
-    HIPCHK(hipMemcpyAsync(ln->din.p, ln->hin.p, pl.in_end, hipMemcpyHostToDevice, ln->st));
+    // the host copy in pieces of about PIPE_BYTES of chunks, each piece's
+    // upload issued as soon as it is in the pinned buffer, so that the copy of
+    // the next piece overlaps it
+    uint8_t *hin = (uint8_t *)ln->hin.p, *din = (uint8_t *)ln->din.p;
+    for (size_t c0 = 0; c0 < pl.C;) {
+        size_t c1 = c0 + 1;
+        while (c1 < pl.C && pl.chunk_dev_off[c1] + lens[c1] - pl.chunk_dev_off[c0] <= PIPE_BYTES) c1++;
+        copy_pieces(hin, pl.chunk_dev_off.data() + c0, ptrs + c0, lens + c0, c1 - c0, copy_threads);
+        const uint64_t a = c0 ? pl.chunk_dev_off[c0] : 0, b = c1 < pl.C ? pl.chunk_dev_off[c1] : pl.in_end;
+        if (b > a) HIPCHK(hipMemcpyAsync(din + a, hin + a, b - a, hipMemcpyHostToDevice, ln->st));
+        c0 = c1;
+    }
     return lane_step_resident(ctx, ln, pl, ln->din.p);
 }
 
@@ -149,16 +163,44 @@ int lane_batch_collect_pinned(Lane *ln, const Plan &pl, const uint8_t **data, st
     return GPZ_OK;
 }
 
+// Results into caller-owned host memory: the download in pieces of
+// PIPE_BYTES through the pinned buffer, each piece copied out while the next
+// ones are still in flight.
 int lane_batch_collect(Lane *ln, const Plan &pl, std::unique_ptr<uint8_t[]> &out, std::vector<uint64_t> &off,
                        std::vector<uint64_t> &len) {
-    const uint8_t *d = nullptr;
-    int rc = lane_batch_collect_pinned(ln, pl, &d, off, len);
-    if (rc) return rc;
+    off.resize(pl.C);
+    len.resize(pl.C);
     uint64_t total = 0;
-    for (size_t c = 0; c < pl.C; c++) total = std::max<uint64_t>(total, off[c] + len[c]);
+    const Work &w = ln->wsp.work;
+    HIPCHK(hipMemcpyAsync(off.data(), w.chunk_off, pl.C * 8, hipMemcpyDeviceToHost, ln->st));
+    HIPCHK(hipMemcpyAsync(len.data(), w.chunk_len, pl.C * 8, hipMemcpyDeviceToHost, ln->st));
+    HIPCHK(hipMemcpyAsync(&total, w.total, 8, hipMemcpyDeviceToHost, ln->st));
+    HIPCHK(hipStreamSynchronize(ln->st));
+    int rc = ln->hout.ensure(total + 8);
+    if (rc) return rc;
     out.reset(new uint8_t[std::max<uint64_t>(total, 1)]);  // no zero fill: every byte is copied
-    copy_bytes(out.get(), d, total);
-    return GPZ_OK;
+    uint8_t *h = (uint8_t *)ln->hout.p;
+    const uint64_t np = (total + PIPE_BYTES - 1) / PIPE_BYTES;
+    std::vector<hipEvent_t> ev(np, nullptr);
+    for (uint64_t k = 0; k < np && !rc; k++) {
+        const uint64_t a = k * PIPE_BYTES, n = std::min(PIPE_BYTES, total - a);
+        if (hipEventCreateWithFlags(&ev[k], hipEventDisableTiming) != hipSuccess ||
+            hipMemcpyAsync(h + a, w.out + a, n, hipMemcpyDeviceToHost, ln->st) != hipSuccess ||
+            hipEventRecord(ev[k], ln->st) != hipSuccess)
+            rc = fail(GPZ_E_DEVICE, "result download failed");
+    }
+    for (uint64_t k = 0; k < np && !rc; k++) {
+        const uint64_t a = k * PIPE_BYTES, n = std::min(PIPE_BYTES, total - a);
+        if (hipEventSynchronize(ev[k]) != hipSuccess) {
+            rc = fail(GPZ_E_DEVICE, "result download failed");
+            break;
+        }
+        copy_bytes(out.get() + a, h + a, n);
+    }
+    if (rc) (void)hipStreamSynchronize(ln->st);
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 }  // namespace gpz
