@@ -1174,6 +1174,8 @@ __global__ __launch_bounds__(K3_T) void k_walk_emit(Work w) {
 
 constexpr int KM_T = 256;     // four waves per workgroup (four members on one wave each, two on two, one on four)
 constexpr uint32_t KM_G = 8;  // blocks of 64 positions per group
+// a group's input bytes are 8 per lane (WalkGroup::r), 512 bytes: at most 8 blocks
+static_assert(KM_G <= 8, "a group's input bytes cover at most 8 blocks");
 constexpr uint32_t HIST_USED = 316;  // litlen 0..285, distance 286..315; 319 takes invalid tokens
 
 // One group of KM_G blocks in registers: lane l's token of each block, of the
